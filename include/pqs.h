@@ -1,0 +1,219 @@
+/*
+ * pqs.h -- C ABI of the MI355X (gfx950) statevector engine behind the
+ * pyramaterised-compatible Python API (package ``pyramaterised_amd``).
+ *
+ * The reference library (pyramaterised 1.0.2) has no native code and no FFI:
+ * its hot path is a Python loop of QuTiP sparse operator x ket products.  Each
+ * entry point below replaces one piece of that path and names the reference
+ * code it stands in for.  A maintainer binds them with ctypes (the binding the
+ * Python package itself uses is pyramaterised_amd/_lib.py).
+ *
+ * Conventions
+ *   - State: 2^N complex amplitudes, interleaved (re, im); complex128 (16 B)
+ *     or complex64 (8 B).  Amplitude index i = sum_k b_k 2^(N-1-k): qubit 0 is
+ *     the most significant bit (QuTiP tensor order).
+ *   - All pointers named *_dev are device pointers (e.g. torch tensors'
+ *     data_ptr()); they are owned by the caller.  The library owns only its
+ *     context workspace.
+ *   - ``stream`` is a hipStream_t passed as void* (NULL = default stream).
+ *     Every call is asynchronous on that stream unless stated otherwise.
+ *   - Every function returns PQS_OK (0) or a negative status; the message of
+ *     the last failure on the calling thread is pqs_last_error().  No C++
+ *     exception crosses the ABI.
+ *   - One context per device per host thread; calls that share a context or
+ *     a program must not run concurrently from several host threads.
+ */
+#ifndef PQS_H
+#define PQS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PQS_ABI_VERSION 1
+
+enum pqs_status {
+  PQS_OK = 0,
+  PQS_ERR_ARG = -1,         /* invalid argument (bad qubit, size, pointer) */
+  PQS_ERR_HIP = -2,         /* a HIP runtime call failed */
+  PQS_ERR_UNSUPPORTED = -3, /* op / size the engine does not support */
+  PQS_ERR_NOMEM = -4,       /* device allocation failed */
+  PQS_ERR_NODEVICE = -5     /* no gfx950 device visible */
+};
+
+enum pqs_dtype { PQS_C128 = 0, PQS_C64 = 1 };
+
+/* Primitive op kinds.  Angles are  scale * params[slot] + offset. */
+enum pqs_op_kind {
+  PQS_OP_MAT1 = 0,   /* fixed 2x2 on q0 (m: 4 complex, row-major)            */
+  PQS_OP_RX = 1,     /* rx(angle) on q0                                      */
+  PQS_OP_RY = 2,     /* ry(angle) on q0                                      */
+  PQS_OP_RZ = 3,     /* rz(angle) on q0                                      */
+  PQS_OP_MAT2 = 4,   /* fixed 4x4 on (q0, q1), q0 = more significant factor  */
+  PQS_OP_CX = 5,     /* CNOT, control q0, target q1                          */
+  PQS_OP_CZ = 6,     /* CZ on (q0, q1)                                       */
+  PQS_OP_RXX = 7,    /* cos(a/2) I - i sin(a/2) X(x)X                        */
+  PQS_OP_RYY = 8,    /* ... Y(x)Y                                            */
+  PQS_OP_RZZ = 9,    /* ... Z(x)Z                                            */
+  PQS_OP_FSIM = 10,  /* fsim(theta = angle(slot), phi = params[slot2])       */
+  /* Derivative ops: applied only by the instance whose tangent index equals
+   * ``tangent`` (a no-op for every other instance).  Consecutive PQS_OP_GEN
+   * terms with the same tangent form one generator D = sum_t coef_t P_t,
+   * P_t = pauli(q0) (x) pauli(q1) (q1 = -1: one qubit; pauli 0=I 1=X 2=Y 3=Z),
+   * coef_t = (m[0], m[1]). */
+  PQS_OP_GEN = 11,
+  /* fSim-family derivative matrices as the reference writes them
+   * (gates.py:609-648, 719-737): pauli selects 1 = d_theta, 2 = d_phi,
+   * 3 = fixed_fSim d_theta; angles from slot / slot2 as for PQS_OP_FSIM. */
+  PQS_OP_GEN_FSIM = 12
+};
+
+typedef struct pqs_op {
+  int32_t kind;
+  int32_t q0, q1;       /* qubits (q1 = -1 for one-qubit ops)                 */
+  int32_t slot, slot2;  /* parameter slots (-1: none)                         */
+  int32_t tangent;      /* derivative ops: tangent index; else -1             */
+  int32_t pauli;        /* PQS_OP_GEN / PQS_OP_GEN_FSIM selector              */
+  int32_t reserved;
+  double scale, offset; /* angle = scale * params[slot] + offset              */
+  double m[32];         /* fixed matrix / coefficient, (re, im) interleaved  */
+} pqs_op;
+
+/* Planner options (zeros select defaults). */
+typedef struct pqs_plan_opts {
+  int32_t tile_bits;    /* qubits resident per tile (<= 13 for c128)          */
+  int32_t low_bits;     /* lowest bit positions forced into every tile        */
+  int32_t block;        /* threads per workgroup (256, 512 or 1024)           */
+  int32_t dtype;        /* pqs_dtype                                          */
+} pqs_plan_opts;
+
+/* One instance of a batched run. */
+typedef struct pqs_instance {
+  void *state_dev;         /* output state (2^N amplitudes)                   */
+  const void *src_dev;     /* input state; NULL: start from the init table    */
+  const double *params_dev;/* parameter vector of this instance               */
+  int32_t tangent;         /* -1: plain evolution; t: insert generator t      */
+  int32_t init;            /* 0: read src (or state if src NULL); 1: |0..0>;
+                              2: product state from the init table            */
+} pqs_instance;
+
+typedef struct pqs_ctx pqs_ctx;
+typedef struct pqs_plan pqs_plan;
+
+int pqs_abi_version(void);
+const char *pqs_last_error(void);
+
+/* Context on one device; owns a workspace that grows on demand. */
+int pqs_ctx_create(int device, pqs_ctx **out);
+int pqs_ctx_destroy(pqs_ctx *ctx);
+int pqs_device_info(pqs_ctx *ctx, int32_t *n_cu, int32_t *max_lds_bytes, char *arch, int32_t arch_len);
+
+/* Compile a flat op list into sweeps (host only; no device needed).
+ * Replaces the per-gate operator rebuild and the Python loop
+ * ``for g in self.gates: circuit_state = g * circuit_state``
+ * (reference circuit.py:118-125, gates.py:122-131). */
+int pqs_plan_create(int32_t n_qubits, const pqs_op *ops, int32_t n_ops,
+                    const pqs_plan_opts *opts, pqs_plan **out);
+int pqs_plan_destroy(pqs_plan *plan);
+/* Plan shape: number of sweeps, tile bits, ops after merging, tangents. */
+int pqs_plan_info(const pqs_plan *plan, int32_t *n_sweeps, int32_t *tile_bits,
+                  int32_t *n_exec_ops, int32_t *n_tangents);
+/* Sweep s: the tile's qubits (ascending bit position; *n_tile entries) and the
+ * indices, into the caller's op array, of the ops it executes in order.
+ * A merged generator reports the index of its first term. */
+int pqs_plan_sweep(const pqs_plan *plan, int32_t s, int32_t *tile_qubits,
+                   int32_t *n_tile, int32_t *op_index, int32_t *n_op);
+/* First sweep that executes the generator of tangent t (-1 if none). */
+int pqs_plan_tangent_sweep(const pqs_plan *plan, int32_t t, int32_t *sweep);
+
+/* Batched evolution: every instance runs the whole plan on its own state.
+ * Replaces PQC.run (circuit.py:118-125), and with tangent >= 0 the
+ * per-parameter re-simulation of PQC.take_derivative (circuit.py:149-172).
+ * init_table_dev: 2N complex (per qubit the single-qubit state (s0, s1)) for
+ * init == 2, in the plan's dtype. */
+int pqs_run(pqs_ctx *ctx, pqs_plan *plan, const pqs_instance *insts,
+            int64_t n_inst, const void *init_table_dev, void *stream);
+
+/* pqs_run for a strided batch: instance i evolves states_dev + i*state_stride
+ * (amplitudes) with params_dev + i*param_stride, tangent tangents_host[i]
+ * (NULL: all plain), starting from src_dev (shared by all instances, init 0),
+ * |0..0> (init 1) or the product table (init 2).  One launch per sweep for
+ * the whole batch.  Replaces ``[self.QC.run("random") for i in
+ * range(sample_N)]`` (measure.py:132, 247, 356, 407). */
+int pqs_run_batch(pqs_ctx *ctx, pqs_plan *plan, void *states_dev, int64_t state_stride,
+                  int64_t batch, const double *params_dev, int64_t param_stride,
+                  const int32_t *tangents_host, int32_t init, const void *src_dev,
+                  const void *init_table_dev, void *stream);
+
+/* All derivative states in one pass: psi_dev gets U|psi0>, jac_dev (row t =
+ * tangent t, n_tangents rows of 2^N) gets every d psi / d theta_t.  Tangent t
+ * is spawned from the plain state at the sweep that holds its generator and
+ * carried to the end with the remaining sweeps (forward mode).
+ * Replaces PQC.get_gradients (circuit.py:174-192). */
+int pqs_jacobian(pqs_ctx *ctx, pqs_plan *plan, const double *params_dev,
+                 int32_t init, const void *src_dev, const void *init_table_dev,
+                 void *psi_dev, void *jac_dev, void *stream);
+
+/* out[p] = <a_p|b_p> (complex, dtype of the states) for n_pairs pairs, each
+ * pair given by element offsets (in amplitudes) from base_dev.
+ * Replaces Qobj.overlap (circuit.py:141, measure.py:52,58,135,418). */
+int pqs_inner(pqs_ctx *ctx, int32_t dtype, const void *base_dev, int64_t dim,
+              const int64_t *a_off_dev, const int64_t *b_off_dev, int64_t n_pairs,
+              double *out_dev, void *stream);
+
+/* Per instance and qubit the 2x2 reduced density matrix (rho00, rho11,
+ * Re rho01, Im rho01) into rho_dev[b][k][4] and the Meyer-Wallach
+ * Q = 2(1 - mean_k Tr rho_k^2) into q_dev[b].  States are contiguous with
+ * stride 2^N.  Replaces Measurements.single_Q / Qobj.ptrace
+ * (measure.py:226-249). */
+int pqs_mw(pqs_ctx *ctx, int32_t dtype, const void *states_dev, int32_t n_qubits,
+           int64_t batch, double *rho_dev, double *q_dev, void *stream);
+
+/* Complex Gram: out[i][j] = <a_i|b_j> for rows a_i (m rows) and b_j (n rows)
+ * of length dim, row strides lda / ldb (amplitudes); out is m x n complex128
+ * (re, im).  Used for the QFIM (measure.py:55-63) and all-pairs fidelities
+ * (measure.py:133-136). */
+int pqs_gram(pqs_ctx *ctx, int32_t dtype, const void *a_dev, int64_t lda, int64_t m,
+             const void *b_dev, int64_t ldb, int64_t n, int64_t dim,
+             double *out_dev, void *stream);
+
+/* QFIM from a Jacobian and the state: F_pq = 4 Re(<J_p|J_q> - <J_p|psi><psi|J_q>)
+ * into f_dev (P x P float64).  Replaces Measurements.get_QFI
+ * (measure.py:33-71). */
+int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac_dev, int64_t P,
+             const void *psi_dev, int64_t dim, double *f_dev, void *stream);
+
+/* sum_t coef_t <psi|P_t|psi> for Pauli strings given as qubit masks
+ * (bit q of xmask/zmask = X/Z component on qubit q; Y = both), coef complex
+ * (re, im); one value per instance (contiguous states).  Replaces
+ * qt.expect(H, psi) for Pauli-sum H (circuit.py:132-137). */
+int pqs_expect_pauli(pqs_ctx *ctx, int32_t dtype, const void *states_dev, int32_t n_qubits,
+                     int64_t batch, const uint64_t *xmask_host, const uint64_t *zmask_host,
+                     const double *coef_host, int32_t n_terms, double *out_dev, void *stream);
+
+/* Histogram of values into n_bins int64 counts over the n_bins + 1 ascending
+ * edges (device array, e.g. numpy.linspace(0, 1, n_bins + 1)) with
+ * numpy.histogram's rules: values outside [edges[0], edges[n_bins]] are
+ * dropped, the last bin is closed.  Replaces np.histogram in
+ * Measurements._gen_histo (measure.py:153-155). */
+int pqs_histogram(pqs_ctx *ctx, const double *values_dev, int64_t n, const double *edges_dev,
+                  int32_t n_bins, int64_t *counts_dev, void *stream);
+
+/* |z|^2 of complex128 values: out[i] = re^2 + im^2 (fidelity from overlap). */
+int pqs_abs2(pqs_ctx *ctx, const double *z_dev, int64_t n, double *out_dev, void *stream);
+
+/* Stabiliser Renyi sum per instance: out[b] = sum over Pauli strings P of
+ * (<psi|P|psi>^2 / 2^n)^alpha, from which the entropy of magic is
+ * log(out)/(1-alpha) - n log 2 (n <= 13).  Replaces the O(4^n)-memory index /
+ * sign matrices and np.dot of Measurements.renyi_entropy_fast
+ * (measure.py:268-349). */
+int pqs_magic(pqs_ctx *ctx, int32_t dtype, const void *states_dev, int32_t n_qubits,
+              int64_t batch, double alpha, double *out_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PQS_H */

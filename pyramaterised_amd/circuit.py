@@ -1,0 +1,285 @@
+"""``PQC``: the reference's circuit container, evolved by the HIP engine.
+
+Reference: pyramaterised/circuit.py.  The layer / gate bookkeeping, the
+parameter binding order and the random-angle stream are the reference's
+(circuit.py:16-116; gates.py:10).  What changes is state generation: instead
+of ``for g in self.gates: circuit_state = g * circuit_state`` over full QuTiP
+operators (circuit.py:118-125), the flat gate list is compiled once into a
+plan (libpqs ``pqs_plan_create``) and every ``run`` is one batched GPU call;
+``get_gradients`` yields all derivative states in one forward pass
+(``pqs_jacobian``) instead of one full re-simulation per parameter
+(circuit.py:149-192).
+"""
+from copy import deepcopy
+
+import numpy as np
+import torch
+
+from . import engine as E
+from .gates import *  # noqa: F401,F403  (circuit.py:9 re-exports the gate library)
+from . import gates as _G
+from .qobj import Ket, PauliSum, ProductKet, Qobj, basis, expect, tensor
+
+
+class PQC:
+    """An n-qubit, p-layer parameterised quantum circuit (circuit.py:12-198)."""
+
+    def __init__(self, n_qubits, dtype="complex128", tile_bits=0, low_bits=0, block=0):
+        self.n_qubits = n_qubits
+        self.n_layers = 0
+        self.layers = []
+        self.gates = []
+        self.parameterised = []
+        self.n_params = 0
+        if n_qubits >= 2:
+            self.set_H("ZZ")
+        self.initial_state = tensor([basis(2, 0) for _ in range(self.n_qubits)])
+        self.state = self.initial_state
+        self.dtype = E.dtype_code(dtype)
+        self.plan_opts = dict(tile_bits=tile_bits, low_bits=low_bits, block=block)
+        self._angles = None
+        self._plans = {}
+        self.set_gates()
+
+    # ------------------------------------------------------------ set-up
+    def set_H(self, H):
+        """circuit.py:25-33: 'ZZ' -> Z_0 Z_1, else a user operator."""
+        if isinstance(H, str) and H == "ZZ":
+            s = [0] * self.n_qubits
+            s[0] = s[1] = 3
+            self.H = PauliSum(self.n_qubits, {tuple(s): 1.0})
+        else:
+            self.H = H
+
+    def set_initial_state(self, state):
+        """circuit.py:35-36: tensor([state] * n)."""
+        self.initial_state = tensor([state for _ in range(self.n_qubits)])
+
+    def add_layer(self, layer, n=1):
+        """circuit.py:38-43."""
+        for _ in range(n):
+            self.layers.append(deepcopy(layer))
+        self.n_layers += n
+        self.set_gates()
+
+    def set_layer(self, layer, pos):
+        """circuit.py:45-48."""
+        self.layers[pos] = deepcopy(layer)
+        self.set_gates()
+
+    def get_layer(self, pos):
+        return self.layers[pos]
+
+    def set_gates(self):
+        """circuit.py:53-72, including its bookkeeping: ``parameterised`` gets a
+        -1 after every gate (the for-else always runs) and ``n_params`` counts
+        each parameter twice.  The true parameter count is ``len(self.slots)``."""
+        layers = []
+        for i in self.layers:
+            layers = layers + i
+        self.gates = layers
+        self.parameterised = []
+        total = 0
+        for gate in self.gates:
+            cur = gate.param_count
+            total += cur
+            for _ in range(cur):
+                total += 1
+                self.parameterised.append(total)
+            else:
+                self.parameterised.append(-1)
+        self.n_params = total
+        # slot of every parameterised gate, in binding order (circuit.py:89-116)
+        self.slots = []
+        self._gate_slot = {}
+        k = 0
+        for g in self.gates:
+            if g.is_param:
+                self._gate_slot[id(g)] = k
+                self.slots.append(g)
+                k += g.param_count
+        self.n_true_params = k
+        self._plans = {}
+
+    # ------------------------------------------------------------ parameters
+    def get_params(self):
+        """circuit.py:74-84."""
+        angles = []
+        for g in self.gates:
+            if g.param_count == 2:
+                angles.append(g.theta)
+                angles.append(g.phi)
+            elif g.param_count == 1:
+                angles.append(g.theta)
+        return angles
+
+    def set_params(self, angles):
+        """circuit.py:86-116: explicit list, or "random" draws of
+        gates.rng.random(1)[0] * 2 * pi in binding order."""
+        parameterised = [g for g in self.gates if g.is_param]
+        vec = np.zeros(self.n_true_params)
+        k = 0
+        for g in parameterised:
+            if g.param_count == 2:
+                if type(angles) != str:
+                    a1, a2 = angles[k], angles[k + 1]
+                elif angles == "random":
+                    a1 = _G.rng.random(1)[0] * 2 * np.pi
+                    a2 = _G.rng.random(1)[0] * 2 * np.pi
+                else:
+                    raise Exception("No parameters supplied!")
+                g.set_theta(a1)
+                g.set_phi(a2)
+                vec[k], vec[k + 1] = a1, a2
+                k += 2
+            elif g.param_count == 1:
+                if type(angles) != str:
+                    a1 = angles[k]
+                elif angles == "random":
+                    a1 = _G.rng.random(1)[0] * 2 * np.pi
+                else:
+                    raise Exception("No parameters supplied!")
+                g.set_theta(a1)
+                vec[k] = a1
+                k += 1
+        self._angles = vec
+
+    def random_params(self, n_samples):
+        """n_samples parameter vectors from gates.rng: the same numbers that
+        n_samples successive ``run("random")`` calls would draw."""
+        P = self.n_true_params
+        return _G.rng.random(n_samples * P).reshape(n_samples, P) * 2 * np.pi
+
+    # ------------------------------------------------------------ compilation
+    def ops(self, with_generators=False):
+        """Flat primitive-op list in gate order; with generators, each
+        parameterised gate's derivative op(s) follow its primitives."""
+        out = []
+        for g in self.gates:
+            slot = self._gate_slot.get(id(g), -1)
+            out += g.prim_ops(max(slot, 0))
+            if with_generators and g.param_count > 0:
+                for w in range(g.param_count):
+                    out += g.gen_ops(slot + w, slot, which=w + 1)
+        return out
+
+    def plan(self, with_generators=False, dtype=None):
+        dt = self.dtype if dtype is None else E.dtype_code(dtype)
+        key = (with_generators, dt)
+        p = self._plans.get(key)
+        if p is None:
+            p = E.Plan(self.n_qubits, self.ops(with_generators), dtype=dt, **self.plan_opts)
+            self._plans[key] = p
+        return p
+
+    def init(self):
+        st = self.initial_state
+        if isinstance(st, ProductKet) or getattr(st, "factors", None) is not None:
+            return E.Init.product(st.factors)
+        if isinstance(st, Ket):
+            return E.Init.from_vector(st.t)
+        return E.Init.from_vector(np.asarray(st.full()).reshape(-1))
+
+    def _params_vec(self):
+        P = self.n_true_params
+        return self._angles if self._angles is not None and len(self._angles) == P else \
+            np.array(self._gate_angles())
+
+    def _gate_angles(self):
+        """Raw bound angles from the gate objects (inverting the stored
+        transforms of negative_R_z / offset_R_z)."""
+        out = []
+        for g in self.gates:
+            if not g.is_param:
+                continue
+            th, ph = g._angles()
+            out.append(th)
+            if g.param_count == 2:
+                out.append(ph)
+        return out
+
+    # ------------------------------------------------------------ evolution
+    def run(self, angles):
+        """circuit.py:118-125: |psi> = U_G ... U_1 |psi0> (one GPU call)."""
+        self.set_params(angles=angles)
+        states = E.run(self.plan(), self._params_vec(), self.init())
+        return Ket(states[0], self.n_qubits)
+
+    def run_batch(self, params):
+        """Evolve many parameter vectors at once: (S, P) -> (S, 2^N) tensor.
+        The gate objects keep the angles of the last row (as after S calls of
+        ``run``)."""
+        params = np.asarray(params, dtype=np.float64) if not torch.is_tensor(params) else params
+        if len(params) == 0:
+            return torch.empty((0, 2 ** self.n_qubits), dtype=E.CDTYPE[self.dtype], device=E.device())
+        last = params[-1].tolist() if not torch.is_tensor(params) else params[-1].cpu().tolist()
+        self.set_params(last)
+        return E.run(self.plan(), params, self.init())
+
+    def update_state(self, angles):
+        """circuit.py:127-130."""
+        self.state = self.run(angles)
+        return self.state
+
+    def cost(self, angles):
+        """circuit.py:132-137: <psi|H|psi>."""
+        self.state = self.run(angles=angles)
+        return expect(self.H, self.state)
+
+    def fidelity(self, target_state):
+        """circuit.py:139-142: |<psi|phi>|^2."""
+        return float(np.abs(self.state.overlap(target_state)) ** 2)
+
+    def flip_deriv(self):
+        """circuit.py:144-147."""
+        for g in self.gates:
+            if g.is_param:
+                g.flip_pauli()
+
+    # ------------------------------------------------------------ derivatives
+    def _tangent_of(self, g_on, param):
+        for i, g in enumerate(self.gates):
+            if g is g_on:
+                slot = self._gate_slot.get(id(g), -1)
+                if slot < 0:
+                    raise ValueError("gate has no parameter")
+                return slot + (param - 1 if param >= 1 else 0)
+        raise ValueError("gate is not in this circuit")  # list.index (circuit.py:156)
+
+    def take_derivative(self, g_on, param=0):
+        """circuit.py:149-172: the circuit with gate g_on replaced by D*gate,
+        run on the initial state (one GPU run with the generator inserted)."""
+        t = self._tangent_of(g_on, param)
+        out = E.run(self.plan(with_generators=True), self._params_vec(), self.init(), tangents=[t])
+        return Ket(out[0], self.n_qubits)
+
+    def get_gradients(self, method="forward"):
+        """circuit.py:174-192: every derivative state.  ``forward`` spawns all
+        tangents from one forward pass (pqs_jacobian); ``plain`` re-simulates
+        the circuit once per parameter, as the reference does."""
+        _, J = self.jacobian(method=method)
+        return [Ket(J[i], self.n_qubits) for i in range(J.shape[0])]
+
+    def jacobian(self, method="forward"):
+        """(psi, J) with J[t] = d psi / d theta_t as tensors."""
+        plan = self.plan(with_generators=True)
+        if method == "forward":
+            return E.jacobian(plan, self._params_vec(), self.init())
+        if method == "plain":
+            P = self.n_true_params
+            dev = E.device()
+            J = torch.empty((P, 2 ** self.n_qubits), dtype=E.CDTYPE[plan.dtype], device=dev)
+            for t in range(P):
+                E.run(plan, self._params_vec(), self.init(), tangents=[t], out=J[t:t + 1])
+            psi = E.run(self.plan(), self._params_vec(), self.init())[0]
+            return psi, J
+        raise ValueError(f"unknown method {method!r}")
+
+    def __repr__(self):
+        line = f"A {self.n_qubits} qubit, {self.n_layers} layer deep PQC. \n"
+        for count, l in enumerate(self.layers):
+            line += f"Layer {count}: {l} \n"
+        return line
+
+
+__all__ = [n for n in dir(_G) if not n.startswith("_")] + ["PQC", "Qobj", "Ket", "PauliSum", "tensor", "basis"]

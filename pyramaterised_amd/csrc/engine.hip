@@ -1,0 +1,1319 @@
+// HIP kernels for gfx950 (MI355X) and the C ABI entry points of pqs.h.
+//
+// Hot path (reference: the Python loop ``for g in self.gates: circuit_state =
+// g * circuit_state``, circuit.py:118-125, where every gate is a full 2^N x 2^N
+// QuTiP CSR operator rebuilt on each parameter change, gates.py:122-131):
+// a planned circuit runs as a sequence of *sweeps*.  One workgroup owns one
+// tile of 2^k amplitudes of one instance: it gathers the tile from HBM into
+// LDS (the low bit positions are always in the tile, so each wave reads
+// contiguous runs), applies every op the planner put in that sweep, and
+// writes the tile back.  Each sweep costs one read and one write of the state
+// however many gates it fuses.  A batch of instances (random parameter
+// vectors, derivative tangents) shares one launch per sweep: the grid is
+// tiles x instances.
+//
+// Reductions (overlap, Meyer-Wallach, Gram / QFIM, Pauli expectation) are
+// two-stage with a fixed order so that every result is bit-identical whatever
+// else runs in the same batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace pqs {
+void set_error(const std::string &s);
+}
+
+namespace {
+
+using pqs::DevOp;
+using pqs::DevSweep;
+using pqs::DevTerm;
+
+int fail(int code, const std::string &msg) {
+  pqs::set_error(msg);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "%s failed: %s", what, hipGetErrorString(e));
+  pqs::set_error(buf);
+  return PQS_ERR_HIP;
+}
+
+#define HIPCHECK(x)                                   \
+  do {                                                \
+    hipError_t e_ = (x);                              \
+    if (e_ != hipSuccess) return hip_fail(e_, #x);    \
+  } while (0)
+
+// Instance record as the sweep kernel reads it.
+struct InstDev {
+  void *state;          // destination state (and source once started)
+  const void *src;      // source at the start sweep (NULL: state itself / init)
+  const double *params; // parameter vector
+  void *alt;            // second buffer of a ping-pong (Jacobian plain state)
+  int32_t tangent;      // -1 or tangent index
+  int32_t init;         // at the start sweep: 0 read, 1 |0..0>, 2 product table
+  int32_t start_sweep;  // first sweep this instance takes part in
+  int32_t pingpong;     // 1: out-of-place every sweep between state and alt
+};
+
+template <typename R> struct CT;
+template <> struct CT<double> { using V = double2; };
+template <> struct CT<float> { using V = float2; };
+
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) {
+  return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) {
+  return V{a.x + b.x, a.y + b.y};
+}
+// a * conj(b)
+template <typename V>
+__device__ __forceinline__ double2 cmulc(V a, V b) {
+  return double2{double(a.x) * b.x + double(a.y) * b.y, double(a.y) * b.x - double(a.x) * b.y};
+}
+
+__device__ __forceinline__ int ins0(int u, int t) {
+  return ((u >> t) << (t + 1)) | (u & ((1 << t) - 1));
+}
+
+__device__ __forceinline__ double op_angle(const DevOp *op, const double *params) {
+  double a = op->offset;
+  if (op->slot >= 0) a += op->scale * params[op->slot];
+  return a;
+}
+
+// 2x2 complex matrix in double: m[0..3] = m00 m01 m10 m11
+struct M2 {
+  double2 m[4];
+};
+struct M4 {
+  double2 m[16];
+};
+
+__device__ void mat1_of(const DevOp *op, const double *params, M2 &M) {
+  const int kind = op->kind;
+  if (kind == PQS_OP_MAT1) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) M.m[i] = double2{op->m[2 * i], op->m[2 * i + 1]};
+    return;
+  }
+  double s, c;
+  sincos(0.5 * op_angle(op, params), &s, &c);
+  if (kind == PQS_OP_RX) {
+    M.m[0] = double2{c, 0};
+    M.m[1] = double2{0, -s};
+    M.m[2] = double2{0, -s};
+    M.m[3] = double2{c, 0};
+  } else if (kind == PQS_OP_RY) {
+    M.m[0] = double2{c, 0};
+    M.m[1] = double2{-s, 0};
+    M.m[2] = double2{s, 0};
+    M.m[3] = double2{c, 0};
+  } else {  // RZ
+    M.m[0] = double2{c, -s};
+    M.m[1] = double2{0, 0};
+    M.m[2] = double2{0, 0};
+    M.m[3] = double2{c, s};
+  }
+}
+
+__device__ void mat2_of(const DevOp *op, const double *params, M4 &M) {
+  const int kind = op->kind;
+  if (kind == PQS_OP_MAT2) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) M.m[i] = double2{op->m[2 * i], op->m[2 * i + 1]};
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) M.m[i] = double2{0, 0};
+  if (kind == PQS_OP_RXX || kind == PQS_OP_RYY) {
+    double s, c;
+    sincos(0.5 * op_angle(op, params), &s, &c);
+    for (int r = 0; r < 4; ++r) M.m[5 * r] = double2{c, 0};
+    // X(x)X: 00<->11, 01<->10 with +1; Y(x)Y: 00<->11 with -1, 01<->10 with +1
+    double sa = kind == PQS_OP_RXX ? 1.0 : -1.0;
+    M.m[3] = double2{0, -s * sa};
+    M.m[12] = double2{0, -s * sa};
+    M.m[6] = double2{0, -s};
+    M.m[9] = double2{0, -s};
+    return;
+  }
+  // fSim family (gates.py:598-606, 619-627, 640-648, 708-716, 729-737)
+  double th = op_angle(op, params);
+  double ph = op->slot2 >= 0 ? params[op->slot2] : 0.0;
+  double st, ct, sp, cp;
+  sincos(th, &st, &ct);
+  sincos(ph, &sp, &cp);
+  M.m[0] = double2{1, 0};
+  int sel = kind == PQS_OP_FSIM ? 0 : op->pauli;
+  if (sel == 1 || sel == 3) {  // d_theta as written by the reference
+    M.m[5] = double2{-st, 0};
+    M.m[6] = double2{0, -ct};
+    M.m[9] = double2{0, -ct};
+    M.m[10] = double2{-st, 0};
+  } else {
+    M.m[5] = double2{ct, 0};
+    M.m[6] = double2{0, -st};
+    M.m[9] = double2{0, -st};
+    M.m[10] = double2{ct, 0};
+  }
+  if (sel == 3) {
+    M.m[15] = double2{1, 0};
+  } else if (sel == 2) {  // -i e^{-i phi}
+    M.m[15] = double2{-sp, -cp};
+  } else {  // e^{-i phi}
+    M.m[15] = double2{cp, -sp};
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ typename CT<R>::V cvt(double2 a) {
+  return typename CT<R>::V{R(a.x), R(a.y)};
+}
+
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sweep(const DevSweep sw, const DevOp *__restrict__ ops,
+                                                 const DevTerm *__restrict__ terms,
+                                                 const InstDev *__restrict__ insts, int64_t n_active,
+                                                 const typename CT<R>::V *__restrict__ init_tab, int n,
+                                                 int s, int n_sweeps) {
+  using V = typename CT<R>::V;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V *tile = reinterpret_cast<V *>(smem_raw);
+  const int tid = threadIdx.x;
+  const int64_t gid = blockIdx.x;
+  const int64_t inst_idx = gid / sw.n_tiles;
+  const int64_t tile_id = gid - inst_idx * sw.n_tiles;
+  if (inst_idx >= n_active) return;
+  const InstDev in = insts[inst_idx];
+  const int S = 1 << sw.k;
+
+  uint64_t base = 0;
+  for (int r = 0; r < sw.cnruns; ++r)
+    base |= (uint64_t(tile_id >> sw.crun_shift[r]) & ((uint64_t(1) << sw.crun_len[r]) - 1))
+            << sw.crun_pos[r];
+
+  const V *src;
+  V *dst;
+  int init = 0;
+  if (in.pingpong) {
+    dst = ((n_sweeps - 1 - s) & 1) == 0 ? (V *)in.state : (V *)in.alt;
+    if (s == in.start_sweep) {
+      src = (const V *)in.src;
+      init = in.src ? 0 : in.init;
+    } else {
+      src = ((n_sweeps - s) & 1) == 0 ? (const V *)in.state : (const V *)in.alt;
+    }
+  } else {
+    dst = (V *)in.state;
+    if (s == in.start_sweep) {
+      src = in.src ? (const V *)in.src : dst;
+      init = in.src ? 0 : in.init;
+    } else {
+      src = dst;
+    }
+  }
+
+  // ---- gather the tile
+  for (int j = tid; j < S; j += BLOCK) {
+    uint64_t gi = base;
+    for (int r = 0; r < sw.nruns; ++r)
+      gi |= uint64_t((j >> sw.run_shift[r]) & ((1 << sw.run_len[r]) - 1)) << sw.run_pos[r];
+    V v;
+    if (init == 0) {
+      v = src[gi];
+    } else if (init == 1) {
+      v = V{gi == 0 ? R(1) : R(0), R(0)};
+    } else {
+      v = V{R(1), R(0)};
+      for (int q = 0; q < n; ++q) v = cmul(v, init_tab[2 * q + int((gi >> (n - 1 - q)) & 1)]);
+    }
+    tile[j] = v;
+  }
+
+  const double *params = in.params;
+  for (int o = sw.op_begin; o < sw.op_end; ++o) {
+    const DevOp *op = ops + o;
+    const int tg = op->tangent;
+    if (tg >= 0 && tg != in.tangent) continue;
+    const int kind = op->kind;
+    const int t0 = op->t0, t1 = op->t1;
+    __syncthreads();
+    switch (kind) {
+      case PQS_OP_MAT1:
+      case PQS_OP_RX:
+      case PQS_OP_RY:
+      case PQS_OP_RZ: {
+        M2 M;
+        mat1_of(op, params, M);
+        if (t0 < 0) {  // diagonal op on a bit constant over the tile
+          const V d = cvt<R>(M.m[((base >> op->g0) & 1) ? 3 : 0]);
+          for (int j = tid; j < S; j += BLOCK) tile[j] = cmul(tile[j], d);
+        } else if (kind == PQS_OP_RZ) {
+          const V d0 = cvt<R>(M.m[0]), d1 = cvt<R>(M.m[3]);
+          for (int j = tid; j < S; j += BLOCK) tile[j] = cmul(tile[j], ((j >> t0) & 1) ? d1 : d0);
+        } else {
+          const V m00 = cvt<R>(M.m[0]), m01 = cvt<R>(M.m[1]), m10 = cvt<R>(M.m[2]), m11 = cvt<R>(M.m[3]);
+          for (int u = tid; u < (S >> 1); u += BLOCK) {
+            const int j0 = ins0(u, t0), j1 = j0 | (1 << t0);
+            const V a = tile[j0], b = tile[j1];
+            tile[j0] = cadd(cmul(m00, a), cmul(m01, b));
+            tile[j1] = cadd(cmul(m10, a), cmul(m11, b));
+          }
+        }
+        break;
+      }
+      case PQS_OP_CX: {
+        const bool ctl_in = t0 >= 0;
+        const bool ctl_on = !ctl_in && ((base >> op->g0) & 1);
+        if (ctl_in || ctl_on) {
+          for (int u = tid; u < (S >> 1); u += BLOCK) {
+            const int j0 = ins0(u, t1);
+            if (ctl_in && !((j0 >> t0) & 1)) continue;
+            const int j1 = j0 | (1 << t1);
+            const V a = tile[j0];
+            tile[j0] = tile[j1];
+            tile[j1] = a;
+          }
+        }
+        break;
+      }
+      case PQS_OP_CZ:
+      case PQS_OP_RZZ: {
+        V d[4];
+        if (kind == PQS_OP_CZ) {
+          d[0] = d[1] = d[2] = V{R(1), R(0)};
+          d[3] = V{R(-1), R(0)};
+        } else {
+          double s_, c_;
+          sincos(0.5 * op_angle(op, params), &s_, &c_);
+          d[0] = d[3] = V{R(c_), R(-s_)};
+          d[1] = d[2] = V{R(c_), R(s_)};
+        }
+        const int c0 = t0 < 0 ? int((base >> op->g0) & 1) : -1;
+        const int c1 = t1 < 0 ? int((base >> op->g1) & 1) : -1;
+        for (int j = tid; j < S; j += BLOCK) {
+          const int b0 = c0 >= 0 ? c0 : ((j >> t0) & 1);
+          const int b1 = c1 >= 0 ? c1 : ((j >> t1) & 1);
+          tile[j] = cmul(tile[j], d[2 * b0 + b1]);
+        }
+        break;
+      }
+      case PQS_OP_MAT2:
+      case PQS_OP_RXX:
+      case PQS_OP_RYY:
+      case PQS_OP_FSIM:
+      case PQS_OP_GEN_FSIM: {
+        M4 M;
+        mat2_of(op, params, M);
+        V m[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) m[i] = cvt<R>(M.m[i]);
+        const int lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+        for (int u = tid; u < (S >> 2); u += BLOCK) {
+          const int j = ins0(ins0(u, lo), hi);
+          const int jj[4] = {j, j | (1 << t1), j | (1 << t0), j | (1 << t0) | (1 << t1)};
+          V a[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) a[c] = tile[jj[c]];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            V acc = cmul(m[4 * r], a[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(m[4 * r + c], a[c]));
+            tile[jj[r]] = acc;
+          }
+        }
+        break;
+      }
+      case pqs::kPauliSum: {
+        constexpr int MAXPER = (1 << (sizeof(R) == 8 ? 13 : 14)) / BLOCK;
+        V acc[MAXPER];
+        const int nt = op->nterms;
+        const DevTerm *tm = terms + op->term0;
+#pragma unroll
+        for (int m = 0; m < MAXPER; ++m) {
+          const int j = tid + m * BLOCK;
+          V a{R(0), R(0)};
+          if (j < S) {
+            for (int t = 0; t < nt; ++t) {
+              const int jj = j ^ int(tm[t].xm);
+              const int par = __popc(uint32_t(jj) & tm[t].zm_tile) + __popcll(base & tm[t].zm_glob);
+              V p = cmul(V{R(tm[t].cre), R(tm[t].cim)}, tile[jj]);
+              if (par & 1) p = V{-p.x, -p.y};
+              a = cadd(a, p);
+            }
+          }
+          acc[m] = a;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < MAXPER; ++m) {
+          const int j = tid + m * BLOCK;
+          if (j < S) tile[j] = acc[m];
+        }
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  __syncthreads();
+
+  // ---- scatter the tile
+  for (int j = tid; j < S; j += BLOCK) {
+    uint64_t gi = base;
+    for (int r = 0; r < sw.nruns; ++r)
+      gi |= uint64_t((j >> sw.run_shift[r]) & ((1 << sw.run_len[r]) - 1)) << sw.run_pos[r];
+    dst[gi] = tile[j];
+  }
+}
+
+// --------------------------------------------------------------- reductions
+constexpr int RB = 256;  // threads per reduction block
+
+template <int N>
+__device__ void block_sum(double (&v)[N], double *sh /* RB*N doubles */) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < N; ++i) sh[i * RB + tid] = v[i];
+  __syncthreads();
+  for (int w = RB / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) sh[i * RB + tid] += sh[i * RB + tid + w];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = sh[i * RB];
+  __syncthreads();
+}
+
+constexpr int kInnerChunk = 4096;
+
+template <typename R>
+__global__ __launch_bounds__(RB) void k_inner_partial(const typename CT<R>::V *__restrict__ base, int64_t dim,
+                                                      const int64_t *__restrict__ a_off,
+                                                      const int64_t *__restrict__ b_off, int64_t n_chunks,
+                                                      double2 *__restrict__ part) {
+  __shared__ double sh[2 * RB];
+  const int64_t pair = blockIdx.x / n_chunks;
+  const int64_t chunk = blockIdx.x - pair * n_chunks;
+  const auto *a = base + a_off[pair];
+  const auto *b = base + b_off[pair];
+  const int64_t lo = chunk * kInnerChunk;
+  const int64_t hi = lo + kInnerChunk < dim ? lo + kInnerChunk : dim;
+  double v[2] = {0, 0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += RB) {
+    // conj(a) * b
+    const auto x = a[i], y = b[i];
+    v[0] += double(x.x) * y.x + double(x.y) * y.y;
+    v[1] += double(x.x) * y.y - double(x.y) * y.x;
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = double2{v[0], v[1]};
+}
+
+__global__ __launch_bounds__(RB) void k_sum_rows(const double2 *__restrict__ part, int64_t n_chunks,
+                                                 double2 *__restrict__ out) {
+  __shared__ double sh[2 * RB];
+  const int64_t row = blockIdx.x;
+  double v[2] = {0, 0};
+  for (int64_t c = threadIdx.x; c < n_chunks; c += RB) {
+    v[0] += part[row * n_chunks + c].x;
+    v[1] += part[row * n_chunks + c].y;
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) out[row] = double2{v[0], v[1]};
+}
+
+// Meyer-Wallach: per chunk of 2^c amplitudes, per qubit (rho00, rho11, rho01).
+template <typename R>
+__global__ __launch_bounds__(RB) void k_mw_partial(const typename CT<R>::V *__restrict__ states, int n, int c,
+                                                   int64_t n_chunks, double *__restrict__ part) {
+  using V = typename CT<R>::V;
+  __shared__ double sh[4 * RB];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V *tile = reinterpret_cast<V *>(smem_raw);
+  const int64_t b = blockIdx.x / n_chunks;
+  const int64_t chunk = blockIdx.x - b * n_chunks;
+  const int64_t dim = int64_t(1) << n;
+  const V *st = states + b * dim;
+  const int64_t cbase = chunk << c;
+  const int CS = 1 << c;
+  for (int j = threadIdx.x; j < CS; j += RB) tile[j] = st[cbase + j];
+  __syncthreads();
+  double *out = part + (b * n_chunks + chunk) * int64_t(n) * 4;
+  // in-chunk qubits: bit position pos < c  <=>  qubit q = n-1-pos
+  for (int pos = 0; pos < c; ++pos) {
+    double v[4] = {0, 0, 0, 0};
+    for (int u = threadIdx.x; u < (CS >> 1); u += RB) {
+      const int j0 = ins0(u, pos), j1 = j0 | (1 << pos);
+      const V a0 = tile[j0], a1 = tile[j1];
+      v[0] += double(a0.x) * a0.x + double(a0.y) * a0.y;
+      v[1] += double(a1.x) * a1.x + double(a1.y) * a1.y;
+      const double2 r = cmulc(a0, a1);
+      v[2] += r.x;
+      v[3] += r.y;
+    }
+    block_sum<4>(v, sh);
+    if (threadIdx.x == 0) {
+      const int q = n - 1 - pos;
+      out[4 * q + 0] = v[0];
+      out[4 * q + 1] = v[1];
+      out[4 * q + 2] = v[2];
+      out[4 * q + 3] = v[3];
+    }
+  }
+  if (c < n) {
+    double tot[1] = {0};
+    for (int j = threadIdx.x; j < CS; j += RB) tot[0] += double(tile[j].x) * tile[j].x + double(tile[j].y) * tile[j].y;
+    block_sum<1>(tot, sh);
+    for (int pos = c; pos < n; ++pos) {
+      const int bit = int((cbase >> pos) & 1);
+      double v[2] = {0, 0};
+      if (!bit) {
+        const V *pt = st + (cbase | (int64_t(1) << pos));
+        for (int j = threadIdx.x; j < CS; j += RB) {
+          const double2 r = cmulc(tile[j], pt[j]);
+          v[0] += r.x;
+          v[1] += r.y;
+        }
+        block_sum<2>(v, sh);
+      }
+      if (threadIdx.x == 0) {
+        const int q = n - 1 - pos;
+        out[4 * q + 0] = bit ? 0.0 : tot[0];
+        out[4 * q + 1] = bit ? tot[0] : 0.0;
+        out[4 * q + 2] = v[0];
+        out[4 * q + 3] = v[1];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(RB) void k_mw_final(const double *__restrict__ part, int n, int64_t n_chunks,
+                                                 double *__restrict__ rho, double *__restrict__ qout) {
+  __shared__ double sh[4 * RB];
+  const int64_t b = blockIdx.x;
+  double purity = 0;
+  for (int q = 0; q < n; ++q) {
+    double v[4] = {0, 0, 0, 0};
+    for (int64_t ch = threadIdx.x; ch < n_chunks; ch += RB) {
+      const double *p = part + ((b * n_chunks + ch) * n + q) * 4;
+      v[0] += p[0];
+      v[1] += p[1];
+      v[2] += p[2];
+      v[3] += p[3];
+    }
+    block_sum<4>(v, sh);
+    if (threadIdx.x == 0) {
+      if (rho) {
+        rho[(b * n + q) * 4 + 0] = v[0];
+        rho[(b * n + q) * 4 + 1] = v[1];
+        rho[(b * n + q) * 4 + 2] = v[2];
+        rho[(b * n + q) * 4 + 3] = v[3];
+      }
+      purity += v[0] * v[0] + v[1] * v[1] + 2 * (v[2] * v[2] + v[3] * v[3]);
+    }
+  }
+  if (threadIdx.x == 0 && qout) qout[b] = 2 * (1 - purity / n);
+}
+
+// Complex Gram  out[i][j] = sum_d conj(a_i[d]) b_j[d]: 16x16 output tile per
+// block, K split over blockIdx.z, partials reduced in order afterwards.
+constexpr int GT = 16, GK = 16;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_gram_partial(const typename CT<R>::V *__restrict__ A, int64_t lda,
+                                                      int64_t m, const typename CT<R>::V *__restrict__ B,
+                                                      int64_t ldb, int64_t nn, int64_t dim, int64_t kchunk,
+                                                      double2 *__restrict__ part) {
+  using V = typename CT<R>::V;
+  __shared__ V sa[GT][GK + 1];
+  __shared__ V sb[GT][GK + 1];
+  const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
+  const int64_t i0 = int64_t(blockIdx.x) * GT, j0 = int64_t(blockIdx.y) * GT;
+  const int64_t k0 = int64_t(blockIdx.z) * kchunk;
+  const int64_t k1 = k0 + kchunk < dim ? k0 + kchunk : dim;
+  double accr = 0, acci = 0;
+  for (int64_t kk = k0; kk < k1; kk += GK) {
+    // load A rows i0..i0+15, cols kk..kk+15 (thread (ty, tx) loads row ty, col tx)
+    {
+      const int64_t r = i0 + ty, k = kk + tx;
+      sa[ty][tx] = (r < m && k < k1) ? A[r * lda + k] : V{0, 0};
+      const int64_t r2 = j0 + ty;
+      sb[ty][tx] = (r2 < nn && k < k1) ? B[r2 * ldb + k] : V{0, 0};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const V x = sa[ty][k], y = sb[tx][k];
+      accr += double(x.x) * y.x + double(x.y) * y.y;
+      acci += double(x.x) * y.y - double(x.y) * y.x;
+    }
+    __syncthreads();
+  }
+  const int64_t i = i0 + ty, j = j0 + tx;
+  if (i < m && j < nn) part[(int64_t(blockIdx.z) * m + i) * nn + j] = double2{accr, acci};
+}
+
+__global__ void k_gram_reduce(const double2 *__restrict__ part, int64_t mn, int64_t ks, double2 *__restrict__ out) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= mn) return;
+  double2 a{0, 0};
+  for (int64_t z = 0; z < ks; ++z) {
+    a.x += part[z * mn + e].x;
+    a.y += part[z * mn + e].y;
+  }
+  out[e] = a;
+}
+
+__global__ void k_qfim_combine(const double2 *__restrict__ G, const double2 *__restrict__ bvec, int64_t P,
+                               double *__restrict__ F) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= P * P) return;
+  int64_t p = e / P, q = e - p * P;
+  if (p > q) {  // mirror the upper triangle (measure.py:66-70)
+    const int64_t t = p;
+    p = q;
+    q = t;
+  }
+  const double2 g = G[p * P + q];
+  const double2 bp = bvec[p], bq = bvec[q];
+  // Re(conj(bp) * bq)
+  const double rhs = bp.x * bq.x + bp.y * bq.y;
+  F[e] = 4 * (g.x - rhs);
+}
+
+struct PauliTerm {
+  uint64_t x, z;
+  double cre, cim;  // includes i^{#Y}
+};
+
+template <typename R>
+__global__ __launch_bounds__(RB) void k_expect_partial(const typename CT<R>::V *__restrict__ states, int n,
+                                                       int64_t n_chunks, const PauliTerm *__restrict__ terms,
+                                                       int nt, double2 *__restrict__ part) {
+  __shared__ double sh[2 * RB];
+  const int64_t b = blockIdx.x / n_chunks;
+  const int64_t chunk = blockIdx.x - b * n_chunks;
+  const int64_t dim = int64_t(1) << n;
+  const auto *st = states + b * dim;
+  const int64_t lo = chunk * kInnerChunk;
+  const int64_t hi = lo + kInnerChunk < dim ? lo + kInnerChunk : dim;
+  double v[2] = {0, 0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += RB) {
+    const auto a = st[i];
+    for (int t = 0; t < nt; ++t) {
+      const uint64_t ii = uint64_t(i) ^ terms[t].x;
+      const auto y = st[ii];
+      // conj(a) * y
+      double re = double(a.x) * y.x + double(a.y) * y.y;
+      double im = double(a.x) * y.y - double(a.y) * y.x;
+      if (__popcll(ii & terms[t].z) & 1) {
+        re = -re;
+        im = -im;
+      }
+      v[0] += terms[t].cre * re - terms[t].cim * im;
+      v[1] += terms[t].cre * im + terms[t].cim * re;
+    }
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = double2{v[0], v[1]};
+}
+
+__global__ void k_histogram(const double *__restrict__ v, int64_t n, const double *__restrict__ edges, int nb,
+                            unsigned long long *__restrict__ counts) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = v[i];
+  const double lo = edges[0], hi = edges[nb];
+  if (!(x >= lo && x <= hi)) return;
+  // numpy.histogram, uniform-bin path: index from the scaled offset, then
+  // corrected against the explicit edges.
+  const double f = ((x - lo) / (hi - lo)) * nb;
+  long long idx = (long long)f;
+  if (idx == nb) idx -= 1;
+  if (x < edges[idx]) idx -= 1;
+  if (x >= edges[idx + 1] && idx != nb - 1) idx += 1;
+  atomicAdd(counts + idx, 1ull);
+}
+
+__global__ void k_abs2(const double2 *__restrict__ z, int64_t n, double *__restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = z[i].x * z[i].x + z[i].y * z[i].y;
+}
+
+template <typename V>
+__global__ void k_fill_zero(V *p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = V{0, 0};
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Stabiliser Renyi entropy sum (reference measure.py:318-349): for every
+// X-part j, f(m) = conj(c_m) c_{m^j}; a Walsh-Hadamard transform over m gives
+// F(i) = sum_m (-1)^{i.m} f(m) = <psi|X^j Z^i|psi> up to a phase; the block
+// returns sum_i |2^{-n/2} F(i)|^{2 alpha}.  One block per (instance, j).
+template <typename R>
+__global__ __launch_bounds__(RB) void k_magic_partial(const typename CT<R>::V *__restrict__ states, int n,
+                                                      double alpha, double *__restrict__ part) {
+  __shared__ double sh[RB];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2 *f = reinterpret_cast<double2 *>(smem_raw);
+  const int64_t D = int64_t(1) << n;
+  const int64_t b = blockIdx.x / D;
+  const int64_t j = blockIdx.x - b * D;
+  const auto *c = states + b * D;
+  for (int64_t m = threadIdx.x; m < D; m += RB) {
+    const auto x = c[m], y = c[m ^ j];
+    f[m] = double2{double(x.x) * y.x + double(x.y) * y.y, double(x.x) * y.y - double(x.y) * y.x};
+  }
+  for (int bit = 0; bit < n; ++bit) {
+    __syncthreads();
+    for (int64_t u = threadIdx.x; u < D / 2; u += RB) {
+      const int64_t i0 = ((u >> bit) << (bit + 1)) | (u & ((int64_t(1) << bit) - 1));
+      const int64_t i1 = i0 | (int64_t(1) << bit);
+      const double2 a = f[i0], d = f[i1];
+      f[i0] = double2{a.x + d.x, a.y + d.y};
+      f[i1] = double2{a.x - d.x, a.y - d.y};
+    }
+  }
+  __syncthreads();
+  const double scale = 1.0 / double(D);  // (2^{-n/2})^2 on |F|^2
+  double v[1] = {0};
+  for (int64_t i = threadIdx.x; i < D; i += RB) {
+    const double a2 = (f[i].x * f[i].x + f[i].y * f[i].y) * scale;
+    v[0] += alpha == 2.0 ? a2 * a2 : (alpha == 0.5 ? sqrt(a2) : pow(a2, alpha));
+  }
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+__global__ __launch_bounds__(RB) void k_sum_rows_real(const double *__restrict__ part, int64_t n_chunks,
+                                                      double *__restrict__ out) {
+  __shared__ double sh[RB];
+  const int64_t row = blockIdx.x;
+  double v[1] = {0};
+  for (int64_t c = threadIdx.x; c < n_chunks; c += RB) v[0] += part[row * n_chunks + c];
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) out[row] = v[0];
+}
+
+inline unsigned grid_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+// ===================================================================== ctx
+struct pqs_ctx {
+  int device = 0;
+  int n_cu = 0;
+  int max_lds = 0;
+  char arch[64] = {0};
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+  void *pin = nullptr;
+  size_t pin_bytes = 0;
+  hipEvent_t pin_event = nullptr;
+  bool pin_pending = false;
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int d) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != d) (void)hipSetDevice(d);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+int ws_reserve(pqs_ctx *ctx, size_t bytes) {
+  if (bytes <= ctx->ws_bytes) return PQS_OK;
+  size_t nb = std::max<size_t>(bytes, size_t(64) << 20);
+  nb = (nb + 4095) & ~size_t(4095);
+  if (ctx->ws) {
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_bytes = 0;
+  }
+  hipError_t e = hipMalloc(&ctx->ws, nb);
+  if (e != hipSuccess) {
+    ctx->ws = nullptr;
+    return fail(PQS_ERR_NOMEM, "workspace allocation failed");
+  }
+  ctx->ws_bytes = nb;
+  return PQS_OK;
+}
+
+// Copy host bytes to device memory asynchronously through pinned staging.
+int upload(pqs_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return PQS_OK;
+  if (ctx->pin_pending) {
+    HIPCHECK(hipEventSynchronize(ctx->pin_event));
+    ctx->pin_pending = false;
+  }
+  if (bytes > ctx->pin_bytes) {
+    if (ctx->pin) HIPCHECK(hipHostFree(ctx->pin));
+    ctx->pin = nullptr;
+    size_t nb = std::max<size_t>(bytes, size_t(1) << 20);
+    HIPCHECK(hipHostMalloc(&ctx->pin, nb, hipHostMallocDefault));
+    ctx->pin_bytes = nb;
+  }
+  std::memcpy(ctx->pin, src, bytes);
+  HIPCHECK(hipMemcpyAsync(dst, ctx->pin, bytes, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipEventRecord(ctx->pin_event, st));
+  ctx->pin_pending = true;
+  return PQS_OK;
+}
+
+int plan_upload(pqs_ctx *ctx, pqs_plan *p) {
+  if (p->d_ops && p->dev_device == ctx->device) return PQS_OK;
+  if (p->d_ops) return fail(PQS_ERR_ARG, "plan already bound to another device");
+  size_t ob = std::max<size_t>(1, p->dev_ops.size()) * sizeof(DevOp);
+  size_t tb = std::max<size_t>(1, p->dev_terms.size()) * sizeof(DevTerm);
+  HIPCHECK(hipMalloc(&p->d_ops, ob));
+  HIPCHECK(hipMalloc(&p->d_terms, tb));
+  if (!p->dev_ops.empty())
+    HIPCHECK(hipMemcpy(p->d_ops, p->dev_ops.data(), p->dev_ops.size() * sizeof(DevOp), hipMemcpyHostToDevice));
+  if (!p->dev_terms.empty())
+    HIPCHECK(hipMemcpy(p->d_terms, p->dev_terms.data(), p->dev_terms.size() * sizeof(DevTerm),
+                       hipMemcpyHostToDevice));
+  p->dev_device = ctx->device;
+  return PQS_OK;
+}
+
+template <typename R, int BLOCK>
+int launch_sweep_t(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64_t n_active,
+                   const void *init_tab, hipStream_t st) {
+  static bool attr_set[64] = {false};
+  const DevSweep &sw = p->dev_sweeps[s];
+  size_t lds = (size_t(1) << sw.k) * sizeof(typename CT<R>::V);
+  if (int(lds) > ctx->max_lds) return fail(PQS_ERR_UNSUPPORTED, "tile does not fit in LDS");
+  auto kern = k_sweep<R, BLOCK>;
+  if (lds > 65536 && ctx->device < 64 && !attr_set[ctx->device]) {
+    HIPCHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+    attr_set[ctx->device] = true;
+  }
+  int64_t grid = sw.n_tiles * n_active;
+  if (grid <= 0) return PQS_OK;
+  if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "grid too large: split the batch");
+  hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(BLOCK), lds, st, sw, (const DevOp *)p->d_ops,
+                     (const DevTerm *)p->d_terms, d_insts, n_active,
+                     (const typename CT<R>::V *)init_tab, p->n, s, int(p->dev_sweeps.size()));
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+int launch_sweep(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64_t n_active,
+                 const void *init_tab, hipStream_t st) {
+  if (p->dtype == PQS_C128) {
+    switch (p->block) {
+      case 256: return launch_sweep_t<double, 256>(ctx, p, s, d_insts, n_active, init_tab, st);
+      case 512: return launch_sweep_t<double, 512>(ctx, p, s, d_insts, n_active, init_tab, st);
+      default: return launch_sweep_t<double, 1024>(ctx, p, s, d_insts, n_active, init_tab, st);
+    }
+  }
+  switch (p->block) {
+    case 256: return launch_sweep_t<float, 256>(ctx, p, s, d_insts, n_active, init_tab, st);
+    case 512: return launch_sweep_t<float, 512>(ctx, p, s, d_insts, n_active, init_tab, st);
+    default: return launch_sweep_t<float, 1024>(ctx, p, s, d_insts, n_active, init_tab, st);
+  }
+}
+
+size_t amp_bytes(int dtype) { return dtype == PQS_C128 ? 16 : 8; }
+
+}  // namespace
+
+extern "C" {
+
+int pqs_ctx_create(int device, pqs_ctx **out) {
+  if (!out) return fail(PQS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(PQS_ERR_NODEVICE, "no HIP device visible");
+  if (device < 0 || device >= count) return fail(PQS_ERR_ARG, "device index out of range");
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    std::string m = std::string("device is ") + prop.gcnArchName + ", the engine is built for gfx950";
+    return fail(PQS_ERR_NODEVICE, m);
+  }
+  DeviceGuard g(device);
+  pqs_ctx *c = new pqs_ctx;
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  c->max_lds = int(prop.sharedMemPerBlock);
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+    c->max_lds = v;
+  std::snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
+  if (hipEventCreateWithFlags(&c->pin_event, hipEventDisableTiming) != hipSuccess) {
+    delete c;
+    return fail(PQS_ERR_HIP, "hipEventCreate failed");
+  }
+  *out = c;
+  return PQS_OK;
+}
+
+int pqs_ctx_destroy(pqs_ctx *ctx) {
+  if (!ctx) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  (void)hipDeviceSynchronize();
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
+  if (ctx->pin_event) (void)hipEventDestroy(ctx->pin_event);
+  delete ctx;
+  return PQS_OK;
+}
+
+int pqs_device_info(pqs_ctx *ctx, int32_t *n_cu, int32_t *max_lds, char *arch, int32_t arch_len) {
+  if (!ctx) return fail(PQS_ERR_ARG, "ctx is NULL");
+  if (n_cu) *n_cu = ctx->n_cu;
+  if (max_lds) *max_lds = ctx->max_lds;
+  if (arch && arch_len > 0) std::snprintf(arch, size_t(arch_len), "%s", ctx->arch);
+  return PQS_OK;
+}
+
+int pqs_plan_destroy(pqs_plan *p) {
+  if (!p) return PQS_OK;
+  if (p->d_ops || p->d_terms) {
+    DeviceGuard g(p->dev_device);
+    (void)hipDeviceSynchronize();
+    if (p->d_ops) (void)hipFree(p->d_ops);
+    if (p->d_terms) (void)hipFree(p->d_terms);
+  }
+  delete p;
+  return PQS_OK;
+}
+
+int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst, const void *init_tab,
+            void *stream) {
+  if (!ctx || !p) return fail(PQS_ERR_ARG, "NULL ctx or plan");
+  if (n_inst < 0 || (n_inst > 0 && !insts)) return fail(PQS_ERR_ARG, "bad instance array");
+  if (n_inst == 0) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<InstDev> h((size_t)n_inst);
+  for (int64_t i = 0; i < n_inst; ++i) {
+    const pqs_instance &a = insts[i];
+    if (!a.state_dev) return fail(PQS_ERR_ARG, "instance without a state buffer");
+    if (a.init < 0 || a.init > 2) return fail(PQS_ERR_ARG, "instance init must be 0, 1 or 2");
+    if (a.init == 2 && !init_tab) return fail(PQS_ERR_ARG, "product init needs an init table");
+    if (a.tangent >= p->n_tangents) return fail(PQS_ERR_ARG, "tangent index beyond the plan's tangents");
+    InstDev d{};
+    d.state = a.state_dev;
+    d.src = a.src_dev;
+    d.params = a.params_dev;
+    d.alt = nullptr;
+    d.tangent = a.tangent < 0 ? -1 : a.tangent;
+    d.init = a.init;
+    d.start_sweep = 0;
+    d.pingpong = 0;
+    h[size_t(i)] = d;
+  }
+  int rc = plan_upload(ctx, p);
+  if (rc) return rc;
+  size_t tb = h.size() * sizeof(InstDev);
+  rc = ws_reserve(ctx, tb);
+  if (rc) return rc;
+  rc = upload(ctx, ctx->ws, h.data(), tb, st);
+  if (rc) return rc;
+  for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
+    rc = launch_sweep(ctx, p, s, (const InstDev *)ctx->ws, n_inst, init_tab, st);
+    if (rc) return rc;
+  }
+  return PQS_OK;
+}
+
+int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride, int64_t batch,
+                  const double *params, int64_t param_stride, const int32_t *tangents, int32_t init,
+                  const void *src, const void *init_tab, void *stream) {
+  if (!ctx || !p || !states) return fail(PQS_ERR_ARG, "NULL ctx, plan or states");
+  if (batch < 0) return fail(PQS_ERR_ARG, "negative batch");
+  if (batch == 0) return PQS_OK;
+  if (state_stride < (int64_t(1) << p->n)) return fail(PQS_ERR_ARG, "state stride below 2^N");
+  if (init < 0 || init > 2) return fail(PQS_ERR_ARG, "init must be 0, 1 or 2");
+  if (init == 2 && !init_tab) return fail(PQS_ERR_ARG, "product init needs an init table");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ab = amp_bytes(p->dtype);
+  std::vector<InstDev> h((size_t)batch);
+  for (int64_t i = 0; i < batch; ++i) {
+    InstDev &d = h[(size_t)i];
+    d.state = (char *)states + size_t(i) * size_t(state_stride) * ab;
+    d.src = src;
+    d.params = params ? params + i * param_stride : nullptr;
+    d.alt = nullptr;
+    d.tangent = tangents ? tangents[i] : -1;
+    if (d.tangent >= p->n_tangents) return fail(PQS_ERR_ARG, "tangent index beyond the plan's tangents");
+    if (d.tangent < 0) d.tangent = -1;
+    d.init = init;
+    d.start_sweep = 0;
+    d.pingpong = 0;
+  }
+  int rc = plan_upload(ctx, p);
+  if (rc) return rc;
+  const size_t tb = h.size() * sizeof(InstDev);
+  rc = ws_reserve(ctx, tb);
+  if (rc) return rc;
+  rc = upload(ctx, ctx->ws, h.data(), tb, st);
+  if (rc) return rc;
+  for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
+    rc = launch_sweep(ctx, p, s, (const InstDev *)ctx->ws, batch, init_tab, st);
+    if (rc) return rc;
+  }
+  return PQS_OK;
+}
+
+int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, const void *src,
+                 const void *init_tab, void *psi, void *jac, void *stream) {
+  if (!ctx || !p || !psi) return fail(PQS_ERR_ARG, "NULL ctx, plan or psi");
+  if (p->n_tangents > 0 && !jac) return fail(PQS_ERR_ARG, "NULL jacobian buffer");
+  if (init < 0 || init > 2) return fail(PQS_ERR_ARG, "init must be 0, 1 or 2");
+  if (init == 0 && !src) return fail(PQS_ERR_ARG, "init 0 needs a source state");
+  if (init == 2 && !init_tab) return fail(PQS_ERR_ARG, "product init needs an init table");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int S = int(p->dev_sweeps.size());
+  const int64_t dim = int64_t(1) << p->n;
+  const size_t ab = amp_bytes(p->dtype);
+  int rc = plan_upload(ctx, p);
+  if (rc) return rc;
+  // workspace: [alt state | instance table]
+  const size_t alt_b = (S > 1) ? size_t(dim) * ab : 0;
+  const size_t alt_off = 0;
+  const size_t tab_off = (alt_b + 255) & ~size_t(255);
+  const size_t tb = size_t(1 + p->n_tangents) * sizeof(InstDev);
+  rc = ws_reserve(ctx, tab_off + tb);
+  if (rc) return rc;
+  char *ws = (char *)ctx->ws;
+  void *alt = S > 1 ? (void *)(ws + alt_off) : nullptr;
+  auto W = [&](int s) -> void * { return ((S - 1 - s) & 1) == 0 ? psi : alt; };
+
+  std::vector<int> order;
+  for (int t = 0; t < p->n_tangents; ++t)
+    if (p->tangent_sweep[t] >= 0) order.push_back(t);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return p->tangent_sweep[a] < p->tangent_sweep[b]; });
+  std::vector<InstDev> h;
+  InstDev pr{};
+  pr.state = psi;
+  pr.alt = alt;
+  pr.src = init == 0 ? src : nullptr;
+  pr.params = params;
+  pr.tangent = -1;
+  pr.init = init;
+  pr.start_sweep = 0;
+  pr.pingpong = S > 1 ? 1 : 0;
+  h.push_back(pr);
+  for (int t : order) {
+    InstDev d{};
+    const int s0 = p->tangent_sweep[t];
+    d.state = (char *)jac + size_t(t) * size_t(dim) * ab;
+    d.params = params;
+    d.tangent = t;
+    d.start_sweep = s0;
+    d.pingpong = 0;
+    if (s0 == 0) {
+      d.src = init == 0 ? src : nullptr;
+      d.init = init;
+    } else {
+      d.src = W(s0 - 1);
+      d.init = 0;
+    }
+    h.push_back(d);
+  }
+  // tangents without a generator: zero rows
+  for (int t = 0; t < p->n_tangents; ++t) {
+    if (p->tangent_sweep[t] >= 0) continue;
+    void *row = (char *)jac + size_t(t) * size_t(dim) * ab;
+    if (p->dtype == PQS_C128)
+      hipLaunchKernelGGL(k_fill_zero<double2>, dim3(grid_for(dim, 256)), dim3(256), 0, st, (double2 *)row, dim);
+    else
+      hipLaunchKernelGGL(k_fill_zero<float2>, dim3(grid_for(dim, 256)), dim3(256), 0, st, (float2 *)row, dim);
+    HIPCHECK(hipGetLastError());
+  }
+  InstDev *d_tab = (InstDev *)(ws + tab_off);
+  rc = upload(ctx, d_tab, h.data(), h.size() * sizeof(InstDev), st);
+  if (rc) return rc;
+  size_t active = 1;
+  for (int s = 0; s < S; ++s) {
+    while (active < h.size() && h[active].start_sweep <= s) ++active;
+    rc = launch_sweep(ctx, p, s, d_tab, int64_t(active), init_tab, st);
+    if (rc) return rc;
+  }
+  return PQS_OK;
+}
+
+int pqs_inner(pqs_ctx *ctx, int32_t dtype, const void *base, int64_t dim, const int64_t *a_off,
+              const int64_t *b_off, int64_t n_pairs, double *out, void *stream) {
+  if (!ctx || !base || !a_off || !b_off || !out) return fail(PQS_ERR_ARG, "NULL argument");
+  if (dim <= 0 || n_pairs < 0) return fail(PQS_ERR_ARG, "bad sizes");
+  if (n_pairs == 0) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nch = (dim + kInnerChunk - 1) / kInnerChunk;
+  int rc = ws_reserve(ctx, size_t(n_pairs * nch) * sizeof(double2));
+  if (rc) return rc;
+  double2 *part = (double2 *)ctx->ws;
+  const int64_t grid = n_pairs * nch;
+  if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "too many pairs x chunks");
+  if (dtype == PQS_C128)
+    hipLaunchKernelGGL(k_inner_partial<double>, dim3(unsigned(grid)), dim3(RB), 0, st, (const double2 *)base, dim,
+                       a_off, b_off, nch, part);
+  else
+    hipLaunchKernelGGL(k_inner_partial<float>, dim3(unsigned(grid)), dim3(RB), 0, st, (const float2 *)base, dim,
+                       a_off, b_off, nch, part);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_rows, dim3(unsigned(n_pairs)), dim3(RB), 0, st, part, nch, (double2 *)out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+int pqs_mw(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t batch, double *rho, double *q,
+           void *stream) {
+  if (!ctx || !states) return fail(PQS_ERR_ARG, "NULL argument");
+  if (n < 1 || n > pqs::kMaxQubits || batch < 0) return fail(PQS_ERR_ARG, "bad sizes");
+  if (batch == 0) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int c = std::min(n, 11);
+  const int64_t nch = int64_t(1) << (n - c);
+  int rc = ws_reserve(ctx, size_t(batch * nch * n * 4) * sizeof(double));
+  if (rc) return rc;
+  double *part = (double *)ctx->ws;
+  const int64_t grid = batch * nch;
+  if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "batch too large");
+  const size_t lds = (size_t(1) << c) * amp_bytes(dtype);
+  if (dtype == PQS_C128)
+    hipLaunchKernelGGL(k_mw_partial<double>, dim3(unsigned(grid)), dim3(RB), lds, st, (const double2 *)states, n, c,
+                       nch, part);
+  else
+    hipLaunchKernelGGL(k_mw_partial<float>, dim3(unsigned(grid)), dim3(RB), lds, st, (const float2 *)states, n, c,
+                       nch, part);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_mw_final, dim3(unsigned(batch)), dim3(RB), 0, st, part, n, nch, rho, q);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// K split of a Gram product: enough blocks to fill the device.
+void gram_split(const pqs_ctx *ctx, int64_t m, int64_t nn, int64_t dim, int64_t &ks, int64_t &kchunk) {
+  const int64_t tm = (m + GT - 1) / GT, tn = (nn + GT - 1) / GT;
+  ks = std::max<int64_t>(1, (4 * int64_t(ctx->n_cu) + tm * tn - 1) / (tm * tn));
+  ks = std::min<int64_t>(ks, (dim + 255) / 256);
+  ks = std::min<int64_t>(ks, 65535);
+  kchunk = (dim + ks - 1) / ks;
+  kchunk = (kchunk + GK - 1) / GK * GK;
+  ks = (dim + kchunk - 1) / kchunk;
+}
+
+size_t gram_part_bytes(const pqs_ctx *ctx, int64_t m, int64_t nn, int64_t dim) {
+  int64_t ks, kc;
+  gram_split(ctx, m, nn, dim, ks, kc);
+  return size_t(ks * m * nn) * sizeof(double2);
+}
+
+// Gram into ``out`` with partials at ``part`` (>= gram_part_bytes).
+int gram_impl(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m, const void *b, int64_t ldb,
+              int64_t nn, int64_t dim, double2 *out, double2 *part, hipStream_t st) {
+  int64_t ks, kchunk;
+  gram_split(ctx, m, nn, dim, ks, kchunk);
+  const int64_t tm = (m + GT - 1) / GT, tn = (nn + GT - 1) / GT;
+  if (tm > 0x7fffffff || tn > 65535) return fail(PQS_ERR_UNSUPPORTED, "Gram too large");
+  dim3 grid((unsigned)tm, (unsigned)tn, (unsigned)ks);
+  if (dtype == PQS_C128)
+    hipLaunchKernelGGL(k_gram_partial<double>, grid, dim3(256), 0, st, (const double2 *)a, lda, m,
+                       (const double2 *)b, ldb, nn, dim, kchunk, part);
+  else
+    hipLaunchKernelGGL(k_gram_partial<float>, grid, dim3(256), 0, st, (const float2 *)a, lda, m, (const float2 *)b,
+                       ldb, nn, dim, kchunk, part);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(m * nn, 256)), dim3(256), 0, st, part, m * nn, ks, out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pqs_gram(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m, const void *b, int64_t ldb,
+             int64_t nn, int64_t dim, double *out, void *stream) {
+  if (!ctx || !a || !b || !out) return fail(PQS_ERR_ARG, "NULL argument");
+  if (m <= 0 || nn <= 0 || dim <= 0 || lda < dim || ldb < dim) return fail(PQS_ERR_ARG, "bad sizes");
+  DeviceGuard g(ctx->device);
+  int rc = ws_reserve(ctx, gram_part_bytes(ctx, m, nn, dim));
+  if (rc) return rc;
+  return gram_impl(ctx, dtype, a, lda, m, b, ldb, nn, dim, (double2 *)out, (double2 *)ctx->ws,
+                   (hipStream_t)stream);
+}
+
+int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void *psi, int64_t dim, double *F,
+             void *stream) {
+  if (!ctx || !jac || !psi || !F) return fail(PQS_ERR_ARG, "NULL argument");
+  if (P <= 0 || dim <= 0) return fail(PQS_ERR_ARG, "bad sizes");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  // workspace: [partials | G (P x P) | b (P)]
+  const size_t part_b = std::max(gram_part_bytes(ctx, P, P, dim), gram_part_bytes(ctx, 1, P, dim));
+  const size_t off_g = (part_b + 255) & ~size_t(255);
+  const size_t off_b = (off_g + size_t(P * P) * sizeof(double2) + 255) & ~size_t(255);
+  int rc = ws_reserve(ctx, off_b + size_t(P) * sizeof(double2));
+  if (rc) return rc;
+  char *ws = (char *)ctx->ws;
+  double2 *part = (double2 *)ws;
+  double2 *G = (double2 *)(ws + off_g);
+  double2 *bv = (double2 *)(ws + off_b);
+  rc = gram_impl(ctx, dtype, jac, dim, P, jac, dim, P, dim, G, part, st);
+  if (rc) return rc;
+  rc = gram_impl(ctx, dtype, psi, dim, 1, jac, dim, P, dim, bv, part, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_qfim_combine, dim3(grid_for(P * P, 256)), dim3(256), 0, st, G, bv, P, F);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+int pqs_expect_pauli(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t batch, const uint64_t *xm,
+                     const uint64_t *zm, const double *coef, int32_t nt, double *out, void *stream) {
+  if (!ctx || !states || !out || (nt > 0 && (!xm || !zm || !coef))) return fail(PQS_ERR_ARG, "NULL argument");
+  if (n < 1 || n > pqs::kMaxQubits || batch < 0 || nt < 0) return fail(PQS_ERR_ARG, "bad sizes");
+  if (batch == 0) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t dim = int64_t(1) << n;
+  const int64_t nch = (dim + kInnerChunk - 1) / kInnerChunk;
+  std::vector<PauliTerm> h(size_t(std::max(nt, 1)));
+  for (int t = 0; t < nt; ++t) {
+    // convert qubit masks (bit q = qubit q) to index masks (bit n-1-q)
+    uint64_t x = 0, z = 0;
+    int ny = 0;
+    for (int q = 0; q < n; ++q) {
+      const bool bx = (xm[t] >> q) & 1, bz = (zm[t] >> q) & 1;
+      if (bx) x |= uint64_t(1) << (n - 1 - q);
+      if (bz) z |= uint64_t(1) << (n - 1 - q);
+      if (bx && bz) ++ny;
+    }
+    double cr = coef[2 * t], ci = coef[2 * t + 1];
+    for (int r = 0; r < ny; ++r) {
+      const double t2 = cr;
+      cr = -ci;
+      ci = t2;
+    }
+    h[size_t(t)] = PauliTerm{x, z, cr, ci};
+  }
+  const size_t part_b = size_t(batch * nch) * sizeof(double2);
+  const size_t off_t = (part_b + 255) & ~size_t(255);
+  int rc = ws_reserve(ctx, off_t + h.size() * sizeof(PauliTerm));
+  if (rc) return rc;
+  char *ws = (char *)ctx->ws;
+  PauliTerm *d_terms = (PauliTerm *)(ws + off_t);
+  rc = upload(ctx, d_terms, h.data(), h.size() * sizeof(PauliTerm), st);
+  if (rc) return rc;
+  const int64_t grid = batch * nch;
+  if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "batch too large");
+  if (dtype == PQS_C128)
+    hipLaunchKernelGGL(k_expect_partial<double>, dim3(unsigned(grid)), dim3(RB), 0, st, (const double2 *)states, n,
+                       nch, d_terms, nt, (double2 *)ws);
+  else
+    hipLaunchKernelGGL(k_expect_partial<float>, dim3(unsigned(grid)), dim3(RB), 0, st, (const float2 *)states, n,
+                       nch, d_terms, nt, (double2 *)ws);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_rows, dim3(unsigned(batch)), dim3(RB), 0, st, (const double2 *)ws, nch, (double2 *)out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+int pqs_histogram(pqs_ctx *ctx, const double *values, int64_t n, const double *edges, int32_t nb,
+                  int64_t *counts, void *stream) {
+  if (!ctx || !edges || !counts || (n > 0 && !values)) return fail(PQS_ERR_ARG, "NULL argument");
+  if (nb < 1 || n < 0) return fail(PQS_ERR_ARG, "bad sizes");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(nb, 256)), dim3(256), 0, st, (unsigned long long *)counts,
+                     int64_t(nb));
+  HIPCHECK(hipGetLastError());
+  if (n > 0) {
+    hipLaunchKernelGGL(k_histogram, dim3(grid_for(n, 256)), dim3(256), 0, st, values, n, edges, nb,
+                       (unsigned long long *)counts);
+    HIPCHECK(hipGetLastError());
+  }
+  return PQS_OK;
+}
+
+int pqs_abs2(pqs_ctx *ctx, const double *z, int64_t n, double *out, void *stream) {
+  if (!ctx || (n > 0 && (!z || !out))) return fail(PQS_ERR_ARG, "NULL argument");
+  if (n <= 0) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  hipLaunchKernelGGL(k_abs2, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, (const double2 *)z, n,
+                     out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+int pqs_magic(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t batch, double alpha,
+              double *out, void *stream) {
+  if (!ctx || !states || !out) return fail(PQS_ERR_ARG, "NULL argument");
+  if (n < 1 || batch < 0) return fail(PQS_ERR_ARG, "bad sizes");
+  if (n > 13) return fail(PQS_ERR_UNSUPPORTED, "stabiliser entropy is limited to 13 qubits (4^n work)");
+  if (batch == 0) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t D = int64_t(1) << n;
+  int rc = ws_reserve(ctx, size_t(batch * D) * sizeof(double));
+  if (rc) return rc;
+  const size_t lds = size_t(D) * sizeof(double2);
+  if (int(lds) > ctx->max_lds) return fail(PQS_ERR_UNSUPPORTED, "state does not fit in LDS");
+  static bool attr[2] = {false, false};
+  if (lds > 65536) {
+    if (!attr[0]) {
+      HIPCHECK(hipFuncSetAttribute((const void *)k_magic_partial<double>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+      HIPCHECK(hipFuncSetAttribute((const void *)k_magic_partial<float>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+      attr[0] = true;
+    }
+  }
+  const int64_t grid = batch * D;
+  if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "batch too large");
+  double *part = (double *)ctx->ws;
+  if (dtype == PQS_C128)
+    hipLaunchKernelGGL(k_magic_partial<double>, dim3(unsigned(grid)), dim3(RB), lds, st, (const double2 *)states, n,
+                       alpha, part);
+  else
+    hipLaunchKernelGGL(k_magic_partial<float>, dim3(unsigned(grid)), dim3(RB), lds, st, (const float2 *)states, n,
+                       alpha, part);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_rows_real, dim3(unsigned(batch)), dim3(RB), 0, st, part, D, out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+}  // extern "C"

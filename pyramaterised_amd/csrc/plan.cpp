@@ -1,0 +1,391 @@
+// Planner: turns a flat op list (the reference's ``PQC.gates`` order, one
+// primitive per local gate; reference circuit.py:53-61,118-125) into sweeps.
+//
+// A sweep keeps 2^k amplitudes of one tile in LDS: k "tile qubits" T, the
+// rest fixed per tile.  An op can run in a sweep when every qubit it acts on
+// non-diagonally is in T; qubits it touches only diagonally (a CNOT control,
+// both CZ / R_zz qubits, R_z) may lie outside T because their bit is constant
+// across the tile.  Ops keep the relative order of every pair that shares a
+// qubit on which at least one of them is non-diagonal; all other pairs
+// commute (both are block-diagonal in that qubit's basis), so reordering them
+// does not change the linear map.  The greedy pass below grows T from the
+// earliest dependency-free ops, so consecutive layers of 1- and 2-qubit gates
+// fuse into as few full-state sweeps as the tile size allows.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "plan.hpp"
+
+namespace pqs {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &s) { g_err = s; }
+std::string plan_error() { return g_err; }
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+static inline int popc(qmask m) { return __builtin_popcountll(m); }
+static inline qmask bitq(int q) { return qmask(1) << q; }
+
+static bool is_two_qubit(int kind) {
+  switch (kind) {
+    case PQS_OP_MAT2: case PQS_OP_CX: case PQS_OP_CZ: case PQS_OP_RXX:
+    case PQS_OP_RYY: case PQS_OP_RZZ: case PQS_OP_FSIM: case PQS_OP_GEN_FSIM:
+      return true;
+    default:
+      return false;
+  }
+}
+
+static bool has_angle(int kind) {
+  switch (kind) {
+    case PQS_OP_RX: case PQS_OP_RY: case PQS_OP_RZ: case PQS_OP_RXX:
+    case PQS_OP_RYY: case PQS_OP_RZZ: case PQS_OP_FSIM: case PQS_OP_GEN_FSIM:
+      return true;
+    default:
+      return false;
+  }
+}
+
+static int validate(int n, const pqs_op &o, int idx) {
+  char buf[256];
+  auto bad = [&](const char *what) {
+    std::snprintf(buf, sizeof buf, "op %d (kind %d): %s", idx, o.kind, what);
+    return fail(PQS_ERR_ARG, buf);
+  };
+  if (o.kind < PQS_OP_MAT1 || o.kind > PQS_OP_GEN_FSIM) return bad("unknown kind");
+  if (o.q0 < 0 || o.q0 >= n) return bad("q0 out of range");
+  bool two = is_two_qubit(o.kind) || (o.kind == PQS_OP_GEN && o.q1 >= 0);
+  if (two) {
+    if (o.q1 < 0 || o.q1 >= n) return bad("q1 out of range");
+    if (o.q1 == o.q0) return bad("q0 == q1");
+  }
+  if (has_angle(o.kind) && o.slot < 0 && o.scale != 0.0) return bad("angle op without a slot");
+  if (o.kind == PQS_OP_FSIM && o.slot2 < 0) return bad("fSim without a phi slot");
+  if (o.kind == PQS_OP_GEN || o.kind == PQS_OP_GEN_FSIM) {
+    if (o.tangent < 0) return bad("derivative op without a tangent index");
+    if (o.kind == PQS_OP_GEN && (o.pauli < 0 || o.pauli > 3)) return bad("pauli must be 0..3");
+    if (o.kind == PQS_OP_GEN_FSIM && (o.pauli < 1 || o.pauli > 3)) return bad("fSim derivative selector must be 1..3");
+    if (o.kind == PQS_OP_GEN_FSIM && o.pauli != 3 && o.slot2 < 0) return bad("fSim derivative without a phi slot");
+  } else if (o.tangent >= 0) {
+    return bad("only derivative ops carry a tangent index");
+  }
+  return PQS_OK;
+}
+
+static void masks_of(int n, const pqs_op &o, qmask &active, qmask &diag) {
+  (void)n;
+  active = diag = 0;
+  switch (o.kind) {
+    case PQS_OP_MAT1: {
+      bool d = o.m[2] == 0 && o.m[3] == 0 && o.m[4] == 0 && o.m[5] == 0;
+      (d ? diag : active) |= bitq(o.q0);
+      break;
+    }
+    case PQS_OP_RX: case PQS_OP_RY: active |= bitq(o.q0); break;
+    case PQS_OP_RZ: diag |= bitq(o.q0); break;
+    case PQS_OP_MAT2: {
+      bool d = true;
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c)
+          if (r != c && (o.m[2 * (4 * r + c)] != 0 || o.m[2 * (4 * r + c) + 1] != 0)) d = false;
+      (d ? diag : active) |= bitq(o.q0) | bitq(o.q1);
+      break;
+    }
+    case PQS_OP_CX: diag |= bitq(o.q0); active |= bitq(o.q1); break;
+    case PQS_OP_CZ: case PQS_OP_RZZ: diag |= bitq(o.q0) | bitq(o.q1); break;
+    case PQS_OP_RXX: case PQS_OP_RYY: case PQS_OP_FSIM: case PQS_OP_GEN_FSIM:
+      active |= bitq(o.q0) | bitq(o.q1);
+      break;
+    case PQS_OP_GEN: {
+      qmask qs = bitq(o.q0) | (o.q1 >= 0 ? bitq(o.q1) : 0);
+      if (o.pauli == 1 || o.pauli == 2) active |= qs;
+      else if (o.pauli == 3) diag |= qs;
+      break;
+    }
+  }
+}
+
+int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_opts *opts) {
+  if (n < 1 || n > kMaxQubits) return fail(PQS_ERR_ARG, "n_qubits must be in 1..34");
+  if (n_ops < 0 || (n_ops > 0 && !ops)) return fail(PQS_ERR_ARG, "bad op array");
+  pqs_plan_opts o{};
+  if (opts) o = *opts;
+  if (o.dtype != PQS_C128 && o.dtype != PQS_C64) return fail(PQS_ERR_ARG, "dtype must be PQS_C128 or PQS_C64");
+  int kmax = o.dtype == PQS_C128 ? 13 : 14;
+  int k = o.tile_bits > 0 ? o.tile_bits : kmax;
+  if (k > kmax) return fail(PQS_ERR_ARG, "tile_bits exceeds the LDS tile limit (13 for c128, 14 for c64)");
+  if (k < 1) return fail(PQS_ERR_ARG, "tile_bits must be >= 1");
+  k = std::min(k, n);
+  int low = o.low_bits > 0 ? o.low_bits : 5;
+  low = std::min(low, k);
+  int block = o.block > 0 ? o.block : 512;
+  if (block != 256 && block != 512 && block != 1024) return fail(PQS_ERR_ARG, "block must be 256, 512 or 1024");
+
+  p.n = n;
+  p.tile_bits = k;
+  p.low_bits = low;
+  p.block = block;
+  p.dtype = o.dtype;
+  p.src_ops.assign(ops, ops + n_ops);
+  p.exec.clear();
+  p.sweeps.clear();
+  p.n_tangents = 0;
+
+  for (int i = 0; i < n_ops; ++i) {
+    int rc = validate(n, ops[i], i);
+    if (rc) return rc;
+    if (ops[i].tangent >= 0) p.n_tangents = std::max(p.n_tangents, ops[i].tangent + 1);
+  }
+
+  // Merge consecutive generator terms of one tangent into a Pauli sum.
+  for (int i = 0; i < n_ops;) {
+    ExecOp e;
+    e.first = i;
+    masks_of(n, ops[i], e.active, e.diag);
+    if (ops[i].kind == PQS_OP_GEN) {
+      int j = i + 1;
+      while (j < n_ops && ops[j].kind == PQS_OP_GEN && ops[j].tangent == ops[i].tangent) {
+        qmask a, d;
+        masks_of(n, ops[j], a, d);
+        e.active |= a;
+        e.diag |= d;
+        ++j;
+      }
+      e.diag &= ~e.active;
+      e.kind = kPauliSum;
+      e.n_src = j - i;
+      i = j;
+    } else {
+      e.kind = ops[i].kind;
+      ++i;
+    }
+    if (popc(e.active) > k) {
+      char buf[200];
+      std::snprintf(buf, sizeof buf,
+                    "op %d acts non-diagonally on %d qubits, more than the %d-qubit tile",
+                    e.first, popc(e.active), k);
+      return fail(PQS_ERR_UNSUPPORTED, buf);
+    }
+    p.exec.push_back(e);
+  }
+
+  // Positions: pos(q) = n-1-q.  Forced low positions 0..low-1 = qubits n-1..n-low.
+  qmask forced = 0;
+  for (int b = 0; b < low; ++b) forced |= bitq(n - 1 - b);
+  qmask all = (n == 64) ? ~qmask(0) : ((qmask(1) << n) - 1);
+
+  std::vector<int> remaining(p.exec.size());
+  for (size_t i = 0; i < remaining.size(); ++i) remaining[i] = int(i);
+
+  while (!remaining.empty() || p.sweeps.empty()) {
+    qmask T = 0;
+    std::vector<int> run, rest;
+    // First try with the forced low positions in the tile (coalesced
+    // gathers); if they leave no room for the next op, drop them.
+    for (int attempt = 0; attempt < 2 && run.empty(); ++attempt) {
+    T = (n <= k) ? all : (attempt == 0 ? forced : 0);
+    run.clear();
+    rest.clear();
+    qmask blocked_any = 0, blocked_nd = 0;
+    for (int ei : remaining) {
+      const ExecOp &e = p.exec[ei];
+      qmask touched = e.active | e.diag;
+      bool dep = (touched & blocked_nd) || (e.active & blocked_any);
+      bool ok = false;
+      if (!dep) {
+        qmask need = e.active & ~T;
+        if (popc(T | need) <= k) {
+          T |= need;
+          ok = true;
+        }
+      }
+      if (ok) {
+        run.push_back(ei);
+      } else {
+        rest.push_back(ei);
+        blocked_any |= touched;
+        blocked_nd |= e.active;
+      }
+    }
+    if (remaining.empty()) break;
+    }
+    if (run.empty() && !remaining.empty()) return fail(PQS_ERR_UNSUPPORTED, "planner made no progress");
+    // Fill the tile to k qubits with the lowest free positions.
+    for (int b = 0; b < n && popc(T) < k; ++b)
+      if (!(T & bitq(n - 1 - b))) T |= bitq(n - 1 - b);
+    Sweep sw;
+    for (int b = 0; b < n; ++b)
+      if (T & bitq(n - 1 - b)) sw.tile_qubits.push_back(n - 1 - b);
+    sw.ops = run;
+    p.sweeps.push_back(sw);
+    remaining.swap(rest);
+    if (remaining.empty()) break;
+  }
+
+  p.tangent_sweep.assign(p.n_tangents, -1);
+  for (size_t s = 0; s < p.sweeps.size(); ++s)
+    for (int ei : p.sweeps[s].ops) {
+      const pqs_op &so = ops[p.exec[ei].first];
+      if (so.tangent >= 0 && p.tangent_sweep[so.tangent] < 0) p.tangent_sweep[so.tangent] = int(s);
+    }
+
+  // ---- device image
+  p.dev_ops.clear();
+  p.dev_terms.clear();
+  p.dev_sweeps.clear();
+  for (const Sweep &sw : p.sweeps) {
+    DevSweep ds{};
+    int tile_index[kMaxQubits];
+    for (int q = 0; q < n; ++q) tile_index[q] = -1;
+    std::vector<int> tpos, cpos;
+    for (size_t i = 0; i < sw.tile_qubits.size(); ++i) {
+      tile_index[sw.tile_qubits[i]] = int(i);
+      tpos.push_back(n - 1 - sw.tile_qubits[i]);
+    }
+    for (int b = 0; b < n; ++b)
+      if (std::find(tpos.begin(), tpos.end(), b) == tpos.end()) cpos.push_back(b);
+    auto runs = [](const std::vector<int> &pos, int32_t *rp, int32_t *rl, int32_t *rs) {
+      int nr = 0;
+      for (size_t i = 0; i < pos.size();) {
+        size_t j = i + 1;
+        while (j < pos.size() && pos[j] == pos[j - 1] + 1) ++j;
+        rp[nr] = pos[i];
+        rl[nr] = int(j - i);
+        rs[nr] = int(i);
+        ++nr;
+        i = j;
+      }
+      return nr;
+    };
+    ds.k = int(tpos.size());
+    ds.nruns = runs(tpos, ds.run_pos, ds.run_len, ds.run_shift);
+    ds.cnruns = runs(cpos, ds.crun_pos, ds.crun_len, ds.crun_shift);
+    ds.n_tiles = int64_t(1) << cpos.size();
+    ds.op_begin = int(p.dev_ops.size());
+    for (int ei : sw.ops) {
+      const ExecOp &e = p.exec[ei];
+      const pqs_op &so = ops[e.first];
+      DevOp d{};
+      d.kind = e.kind == kPauliSum ? kPauliSum : so.kind;
+      d.tangent = so.tangent;
+      d.slot = so.slot;
+      d.slot2 = so.slot2;
+      d.pauli = so.pauli;
+      d.scale = so.scale;
+      d.offset = so.offset;
+      std::memcpy(d.m, so.m, sizeof d.m);
+      d.g0 = n - 1 - so.q0;
+      d.t0 = tile_index[so.q0];
+      d.g1 = so.q1 >= 0 ? n - 1 - so.q1 : -1;
+      d.t1 = so.q1 >= 0 ? tile_index[so.q1] : -1;
+      if (e.kind == kPauliSum) {
+        d.term0 = int(p.dev_terms.size());
+        d.nterms = e.n_src;
+        for (int t = e.first; t < e.first + e.n_src; ++t) {
+          const pqs_op &g = ops[t];
+          DevTerm dt{};
+          int nq = g.q1 >= 0 ? 2 : 1;
+          int qs[2] = {g.q0, g.q1};
+          int ny = 0;
+          for (int a = 0; a < nq; ++a) {
+            int q = qs[a];
+            bool x = g.pauli == 1 || g.pauli == 2;
+            bool z = g.pauli == 2 || g.pauli == 3;
+            if (g.pauli == 2) ++ny;
+            if (x) dt.xm |= 1u << tile_index[q];  // active qubits are in the tile
+            if (z) {
+              if (tile_index[q] >= 0) dt.zm_tile |= 1u << tile_index[q];
+              else dt.zm_glob |= uint64_t(1) << (n - 1 - q);
+            }
+          }
+          // coef * i^ny
+          double cr = g.m[0], ci = g.m[1];
+          for (int r = 0; r < ny; ++r) {
+            double t2 = cr;
+            cr = -ci;
+            ci = t2;
+          }
+          dt.cre = cr;
+          dt.cim = ci;
+          p.dev_terms.push_back(dt);
+        }
+      }
+      p.dev_ops.push_back(d);
+    }
+    ds.op_end = int(p.dev_ops.size());
+    p.dev_sweeps.push_back(ds);
+  }
+  return PQS_OK;
+}
+
+}  // namespace pqs
+
+extern "C" {
+
+const char *pqs_last_error(void) {
+  static thread_local std::string copy;
+  copy = pqs::g_err;
+  return copy.c_str();
+}
+
+int pqs_abi_version(void) { return PQS_ABI_VERSION; }
+
+int pqs_plan_create(int32_t n_qubits, const pqs_op *ops, int32_t n_ops,
+                    const pqs_plan_opts *opts, pqs_plan **out) {
+  if (!out) return pqs::fail(PQS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  pqs_plan *p = new (std::nothrow) pqs_plan;
+  if (!p) return pqs::fail(PQS_ERR_NOMEM, "host allocation failed");
+  int rc;
+  try {
+    rc = pqs::build_plan(*p, n_qubits, ops, n_ops, opts);
+  } catch (...) {
+    rc = pqs::fail(PQS_ERR_NOMEM, "host allocation failed while planning");
+  }
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return PQS_OK;
+}
+
+int pqs_plan_info(const pqs_plan *p, int32_t *n_sweeps, int32_t *tile_bits,
+                  int32_t *n_exec_ops, int32_t *n_tangents) {
+  if (!p) return pqs::fail(PQS_ERR_ARG, "plan is NULL");
+  if (n_sweeps) *n_sweeps = int32_t(p->sweeps.size());
+  if (tile_bits) *tile_bits = p->tile_bits;
+  if (n_exec_ops) *n_exec_ops = int32_t(p->exec.size());
+  if (n_tangents) *n_tangents = p->n_tangents;
+  return PQS_OK;
+}
+
+int pqs_plan_sweep(const pqs_plan *p, int32_t s, int32_t *tile_qubits, int32_t *n_tile,
+                   int32_t *op_index, int32_t *n_op) {
+  if (!p) return pqs::fail(PQS_ERR_ARG, "plan is NULL");
+  if (s < 0 || s >= int32_t(p->sweeps.size())) return pqs::fail(PQS_ERR_ARG, "sweep index out of range");
+  const pqs::Sweep &sw = p->sweeps[s];
+  if (n_tile) *n_tile = int32_t(sw.tile_qubits.size());
+  if (tile_qubits)
+    for (size_t i = 0; i < sw.tile_qubits.size(); ++i) tile_qubits[i] = sw.tile_qubits[i];
+  if (n_op) *n_op = int32_t(sw.ops.size());
+  if (op_index)
+    for (size_t i = 0; i < sw.ops.size(); ++i) op_index[i] = p->exec[sw.ops[i]].first;
+  return PQS_OK;
+}
+
+int pqs_plan_tangent_sweep(const pqs_plan *p, int32_t t, int32_t *sweep) {
+  if (!p || !sweep) return pqs::fail(PQS_ERR_ARG, "NULL argument");
+  if (t < 0 || t >= p->n_tangents) return pqs::fail(PQS_ERR_ARG, "tangent out of range");
+  *sweep = p->tangent_sweep[t];
+  return PQS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,374 @@
+"""Device layer: contexts, plans and the calls into libpqs on torch tensors.
+
+PyTorch provides device memory and the current HIP stream; all arithmetic of
+the hot path runs in libpqs's HIP kernels.  Every function here requires a
+gfx950 GPU and raises ``EngineError`` otherwise.
+"""
+import ctypes as C
+import threading
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import EngineError, check
+
+_tls = threading.local()
+
+CDTYPE = {L.PQS_C128: torch.complex128, L.PQS_C64: torch.complex64}
+DTYPES = {"complex128": L.PQS_C128, "c128": L.PQS_C128, "complex64": L.PQS_C64, "c64": L.PQS_C64}
+
+
+def dtype_code(dtype):
+    if isinstance(dtype, int):
+        return dtype
+    if dtype in (torch.complex128, np.complex128):
+        return L.PQS_C128
+    if dtype in (torch.complex64, np.complex64):
+        return L.PQS_C64
+    try:
+        return DTYPES[str(dtype)]
+    except KeyError:
+        raise ValueError(f"unsupported state dtype {dtype!r}") from None
+
+
+def device():
+    """The torch device the engine runs on (current CUDA/HIP device)."""
+    if not torch.cuda.is_available():
+        raise EngineError("no GPU visible: the statevector engine runs only on gfx950 (MI355X)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class Context:
+    """libpqs context on one device (one per device per host thread)."""
+
+    def __init__(self, dev_index):
+        self.handle = C.c_void_p()
+        self.device_index = dev_index
+        check(L.lib().pqs_ctx_create(dev_index, C.byref(self.handle)))
+        n_cu, lds = C.c_int32(), C.c_int32()
+        arch = C.create_string_buffer(64)
+        check(L.lib().pqs_device_info(self.handle, C.byref(n_cu), C.byref(lds), arch, 64))
+        self.n_cu, self.max_lds, self.arch = n_cu.value, lds.value, arch.value.decode()
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and h.value:
+            try:
+                L.lib().pqs_ctx_destroy(h)
+            except Exception:
+                pass
+
+
+def context():
+    dev = device()
+    ctxs = getattr(_tls, "ctxs", None)
+    if ctxs is None:
+        ctxs = _tls.ctxs = {}
+    c = ctxs.get(dev.index)
+    if c is None:
+        c = ctxs[dev.index] = Context(dev.index)
+    return c
+
+
+def max_tile_bits(dtype_c):
+    """Largest tile (qubits) whose amplitudes fit in one workgroup's LDS."""
+    lds = context().max_lds
+    amp = 16 if dtype_c == L.PQS_C128 else 8
+    k = 1
+    while (2 ** (k + 1)) * amp <= lds and k + 1 <= (13 if dtype_c == L.PQS_C128 else 14):
+        k += 1
+    return k
+
+
+def stream_ptr():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class Plan:
+    """A compiled op list (pqs_plan).  Host-only to build; runs on the GPU."""
+
+    def __init__(self, n_qubits, ops, dtype=L.PQS_C128, tile_bits=0, low_bits=0, block=0):
+        self.n = int(n_qubits)
+        self.dtype = dtype_code(dtype)
+        if tile_bits == 0 and torch.cuda.is_available():
+            tile_bits = max_tile_bits(self.dtype)
+        self.n_ops = len(ops)
+        arr = (L.Op * max(1, len(ops)))(*ops)
+        opts = L.PlanOpts(tile_bits, low_bits, block, self.dtype)
+        self.handle = C.c_void_p()
+        check(L.lib().pqs_plan_create(self.n, arr, len(ops), C.byref(opts), C.byref(self.handle)))
+        ns, tb, ne, nt = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(L.lib().pqs_plan_info(self.handle, C.byref(ns), C.byref(tb), C.byref(ne), C.byref(nt)))
+        self.n_sweeps, self.tile_bits, self.n_exec, self.n_tangents = ns.value, tb.value, ne.value, nt.value
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and h.value:
+            try:
+                L.lib().pqs_plan_destroy(h)
+            except Exception:
+                pass
+
+    def sweeps(self):
+        """[(tile_qubits, op_indices)] for every sweep (host-side description)."""
+        out = []
+        for s in range(self.n_sweeps):
+            tq = (C.c_int32 * 64)()
+            oi = (C.c_int32 * max(1, self.n_ops))()
+            nt, no = C.c_int32(), C.c_int32()
+            check(L.lib().pqs_plan_sweep(self.handle, s, tq, C.byref(nt), oi, C.byref(no)))
+            out.append((list(tq[:nt.value]), list(oi[:no.value])))
+        return out
+
+    def tangent_sweep(self, t):
+        s = C.c_int32()
+        check(L.lib().pqs_plan_tangent_sweep(self.handle, t, C.byref(s)))
+        return s.value
+
+
+# ------------------------------------------------------------------- init state
+
+class Init:
+    """Initial state of a run: |0..0>, a product state (per-qubit 2-vectors),
+    or a full state vector."""
+
+    def __init__(self, kind, table=None, vector=None):
+        self.kind = kind  # "zero" | "product" | "vector"
+        self.table = table
+        self.vector = vector
+
+    @staticmethod
+    def zero():
+        return Init("zero")
+
+    @staticmethod
+    def product(factors):
+        t = np.asarray([np.asarray(f, dtype=np.complex128).ravel() for f in factors])
+        if t.ndim != 2 or t.shape[1] != 2:
+            raise ValueError("product initial state needs one 2-vector per qubit")
+        if np.allclose(t, np.tile([1, 0], (len(t), 1))):
+            return Init("zero")
+        return Init("product", table=t)
+
+    @staticmethod
+    def from_vector(v):
+        return Init("vector", vector=v)
+
+    def device_args(self, dtype_c, dev):
+        """(init code, src tensor or None, table tensor or None)."""
+        if self.kind == "zero":
+            return 1, None, None
+        if self.kind == "product":
+            tab = torch.as_tensor(self.table.reshape(-1), dtype=CDTYPE[dtype_c], device=dev)
+            return 2, None, tab
+        v = self.vector
+        if not torch.is_tensor(v):
+            v = torch.as_tensor(np.asarray(v).reshape(-1))
+        v = v.to(device=dev, dtype=CDTYPE[dtype_c]).contiguous()
+        return 0, v, None
+
+
+# ------------------------------------------------------------------ entry points
+
+def _params_tensor(params, dev, rows=None):
+    p = params if torch.is_tensor(params) else torch.as_tensor(np.asarray(params, dtype=np.float64))
+    p = p.to(device=dev, dtype=torch.float64).contiguous()
+    if rows is not None and p.dim() == 1:
+        p = p.unsqueeze(0)
+    return p
+
+
+def run(plan, params, init, tangents=None, out=None):
+    """Evolve a batch: params (B, P) or (P,) -> states (B, 2^N).
+
+    tangents: None or a length-B sequence of tangent indices (-1 = plain).
+    """
+    ctx = context()
+    dev = device()
+    p = _params_tensor(params, dev, rows=True)
+    if p.numel() == 0:
+        p = torch.zeros((max(1, p.shape[0]) if p.dim() == 2 else 1, 1), dtype=torch.float64, device=dev)
+    B = p.shape[0]
+    dim = 1 << plan.n
+    cdt = CDTYPE[plan.dtype]
+    if out is None:
+        out = torch.empty((B, dim), dtype=cdt, device=dev)
+    elif out.shape != (B, dim) or out.dtype != cdt or not out.is_contiguous():
+        raise ValueError("out has the wrong shape, dtype or layout")
+    code, src, tab = init.device_args(plan.dtype, dev)
+    tg = None
+    if tangents is not None:
+        tg = (C.c_int32 * B)(*[int(t) for t in tangents])
+    pstride = p.stride(0) if p.shape[0] > 1 else 0
+    check(L.lib().pqs_run_batch(ctx.handle, plan.handle, ptr(out), dim, B, ptr(p), pstride, tg, code, ptr(src),
+                                ptr(tab), stream_ptr()))
+    # keep temporaries alive until the stream has consumed them
+    _keepalive(src, tab, p)
+    return out
+
+
+def jacobian(plan, params, init, psi_out=None, jac_out=None):
+    """psi (2^N,), J (n_tangents, 2^N) in one forward pass."""
+    ctx = context()
+    dev = device()
+    p = _params_tensor(params, dev)
+    if p.numel() == 0:
+        p = torch.zeros(1, dtype=torch.float64, device=dev)
+    dim = 1 << plan.n
+    cdt = CDTYPE[plan.dtype]
+    psi = psi_out if psi_out is not None else torch.empty(dim, dtype=cdt, device=dev)
+    J = jac_out if jac_out is not None else torch.empty((max(plan.n_tangents, 0), dim), dtype=cdt, device=dev)
+    code, src, tab = init.device_args(plan.dtype, dev)
+    check(L.lib().pqs_jacobian(ctx.handle, plan.handle, ptr(p), code, ptr(src), ptr(tab), ptr(psi),
+                               ptr(J) if plan.n_tangents > 0 else C.c_void_p(0), stream_ptr()))
+    _keepalive(src, tab, p)
+    return psi, J
+
+
+def inner_pairs(states, a_idx, b_idx):
+    """<s_a|s_b> for rows of a (B, dim) tensor -> complex128 (n_pairs,)."""
+    ctx = context()
+    dev = states.device
+    if states.dim() == 1:
+        states = states.unsqueeze(0)
+    B, dim = states.shape
+    if not states.is_contiguous():
+        states = states.contiguous()
+    a = torch.as_tensor(np.asarray(a_idx, dtype=np.int64) * dim, device=dev)
+    b = torch.as_tensor(np.asarray(b_idx, dtype=np.int64) * dim, device=dev)
+    n = a.numel()
+    out = torch.empty(n, dtype=torch.complex128, device=dev)
+    if n:
+        check(L.lib().pqs_inner(ctx.handle, dtype_code(states.dtype), ptr(states), dim, ptr(a), ptr(b), n,
+                                ptr(out), stream_ptr()))
+    _keepalive(a, b)
+    return out
+
+
+def inner(x, y):
+    """<x|y> of two state tensors (same length) -> complex."""
+    ctx = context()
+    dev = x.device
+    x = x.reshape(-1).contiguous()
+    y = y.reshape(-1).to(dtype=x.dtype).contiguous()
+    dim = x.numel()
+    # express both as offsets from x's storage
+    esz = x.element_size()
+    off_b = (y.data_ptr() - x.data_ptr())
+    if off_b % esz:
+        stacked = torch.stack([x, y])
+        return inner_pairs(stacked, [0], [1])[0]
+    a = torch.zeros(1, dtype=torch.int64, device=dev)
+    b = torch.full((1,), off_b // esz, dtype=torch.int64, device=dev)
+    out = torch.empty(1, dtype=torch.complex128, device=dev)
+    check(L.lib().pqs_inner(ctx.handle, dtype_code(x.dtype), ptr(x), dim, ptr(a), ptr(b), 1, ptr(out),
+                            stream_ptr()))
+    _keepalive(a, b, y)
+    return out[0]
+
+
+def mw(states, n_qubits):
+    """Reduced 1-qubit density matrices and Meyer-Wallach Q for a batch.
+    Returns (rho (B, N, 4) float64 [r00, r11, Re r01, Im r01], Q (B,))."""
+    ctx = context()
+    if states.dim() == 1:
+        states = states.unsqueeze(0)
+    states = states.contiguous()
+    B = states.shape[0]
+    rho = torch.empty((B, n_qubits, 4), dtype=torch.float64, device=states.device)
+    q = torch.empty(B, dtype=torch.float64, device=states.device)
+    check(L.lib().pqs_mw(ctx.handle, dtype_code(states.dtype), ptr(states), n_qubits, B, ptr(rho), ptr(q),
+                         stream_ptr()))
+    return rho, q
+
+
+def gram(a, b):
+    """out[i, j] = <a_i|b_j> for (m, dim) and (n, dim) tensors -> complex128 (m, n)."""
+    ctx = context()
+    a = a.contiguous() if a.dim() == 2 else a.reshape(1, -1).contiguous()
+    b = b.contiguous() if b.dim() == 2 else b.reshape(1, -1).contiguous()
+    if a.dtype != b.dtype:
+        b = b.to(a.dtype)
+    m, dim = a.shape
+    n = b.shape[0]
+    out = torch.empty((m, n), dtype=torch.complex128, device=a.device)
+    check(L.lib().pqs_gram(ctx.handle, dtype_code(a.dtype), ptr(a), dim, m, ptr(b), dim, n, dim, ptr(out),
+                           stream_ptr()))
+    return out
+
+
+def qfim(J, psi):
+    """F = 4 Re(J J^dag - (J psi^*)(psi J^dag)) -> float64 (P, P)."""
+    ctx = context()
+    J = J.contiguous()
+    P, dim = J.shape
+    psi = psi.reshape(-1).to(J.dtype).contiguous()
+    F = torch.empty((P, P), dtype=torch.float64, device=J.device)
+    if P:
+        check(L.lib().pqs_qfim(ctx.handle, dtype_code(J.dtype), ptr(J), P, ptr(psi), dim, ptr(F), stream_ptr()))
+    return F
+
+
+def expect_pauli(states, n_qubits, terms):
+    """sum_t c_t <psi|P_t|psi> per row; terms = [(coef, xmask, zmask)] with
+    qubit masks (bit q = qubit q).  Returns complex128 (B,)."""
+    ctx = context()
+    if states.dim() == 1:
+        states = states.unsqueeze(0)
+    states = states.contiguous()
+    B = states.shape[0]
+    nt = len(terms)
+    xm = (C.c_uint64 * max(1, nt))(*[int(t[1]) for t in terms])
+    zm = (C.c_uint64 * max(1, nt))(*[int(t[2]) for t in terms])
+    cf = (C.c_double * max(2, 2 * nt))(*[v for t in terms for v in (complex(t[0]).real, complex(t[0]).imag)])
+    out = torch.empty(B, dtype=torch.complex128, device=states.device)
+    check(L.lib().pqs_expect_pauli(ctx.handle, dtype_code(states.dtype), ptr(states), n_qubits, B, xm, zm, cf, nt,
+                                   ptr(out), stream_ptr()))
+    return out
+
+
+def histogram(values, edges):
+    """Counts of float64 device values over numpy-style edges -> int64 (nb,)."""
+    ctx = context()
+    values = values.to(torch.float64).contiguous()
+    e = torch.as_tensor(np.asarray(edges, dtype=np.float64), device=values.device)
+    nb = e.numel() - 1
+    counts = torch.empty(nb, dtype=torch.int64, device=values.device)
+    check(L.lib().pqs_histogram(ctx.handle, ptr(values), values.numel(), ptr(e), nb, ptr(counts), stream_ptr()))
+    _keepalive(e)
+    return counts
+
+
+def magic_sum(states, n_qubits, alpha):
+    """sum_P (<P>^2 / 2^n)^alpha per row -> float64 (B,)."""
+    ctx = context()
+    if states.dim() == 1:
+        states = states.unsqueeze(0)
+    states = states.contiguous()
+    out = torch.empty(states.shape[0], dtype=torch.float64, device=states.device)
+    check(L.lib().pqs_magic(ctx.handle, dtype_code(states.dtype), ptr(states), n_qubits, states.shape[0],
+                            float(alpha), ptr(out), stream_ptr()))
+    return out
+
+
+def abs2(z):
+    ctx = context()
+    z = z.to(torch.complex128).contiguous()
+    out = torch.empty(z.shape, dtype=torch.float64, device=z.device)
+    check(L.lib().pqs_abs2(ctx.handle, ptr(z), z.numel(), ptr(out), stream_ptr()))
+    return out
+
+
+# Temporaries handed to asynchronous kernels must outlive them: the caching
+# allocator may hand their memory to the next tensor as soon as Python drops
+# them.  record_stream marks them as used by the current stream.
+def _keepalive(*ts):
+    s = torch.cuda.current_stream()
+    for t in ts:
+        if t is not None and torch.is_tensor(t) and t.is_cuda:
+            t.record_stream(s)

@@ -1,0 +1,240 @@
+"""Pins of the CPU oracle against the reference's known answers, brute force
+and closed forms.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import circuits as C
+from oracle import gates as OG
+from oracle import measures as M
+from oracle import sim as S
+
+QG_ENERGY = 0.46135870050914374  # tests.py:207-209
+QG_ROUNDED_ANGLES = [3.21587011, 5.97193953, 0.90578156, 5.96054027, 1.9592948,
+                     2.65983852, 5.20060878, 2.571074, 3.45319898, 0.17315902,
+                     4.73446249, 3.38125416]  # tests.py:192-205
+
+
+def zz01(psi, n):
+    """<Z0 Z1> (PQC.set_H('ZZ'), circuit.py:28-31; qt.expect at circuit.py:136)."""
+    idx = np.arange(2 ** n)
+    b0 = (idx >> (n - 1)) & 1
+    b1 = (idx >> (n - 2)) & 1
+    sign = np.where(b0 ^ b1, -1.0, 1.0)
+    return float(np.sum(sign * np.abs(psi) ** 2))
+
+
+def rng_draws(k, seed=1):
+    """gates.rng = default_rng(1) (gates.py:10); one rng.random(1)[0]*2*pi per
+    parameter (circuit.py:112)."""
+    r = np.random.default_rng(seed)
+    return [r.random(1)[0] * 2 * np.pi for _ in range(k)]
+
+
+def test_rng_stream_matches_rounded_test_angles():
+    exact = rng_draws(12)
+    assert np.max(np.abs(np.array(exact) - QG_ROUNDED_ANGLES)) < 1e-8
+    # one (k,) draw is bit-identical to k single draws
+    batch = np.random.default_rng(1).random(12) * 2 * np.pi
+    assert np.array_equal(batch, np.array(exact))
+
+
+def test_qg_energy_golden():
+    n, gates = C.qg_test_circuit()
+    psi = S.run(gates, rng_draws(12), n)
+    assert abs(zz01(psi, n) - QG_ENERGY) < 1e-14
+    psi_r = S.run(gates, QG_ROUNDED_ANGLES, n)
+    assert abs(zz01(psi_r, n) - QG_ENERGY) < 1e-5
+
+
+def test_qg_energy_rejects_wrong_chain_order():
+    """A sequential 0->1->2->3 chain gives a different energy: the golden value
+    pins the CHAIN order of gates.py:368-375."""
+    n, gates = C.qg_test_circuit()
+    seq = S.OGate("seq", 0, lambda t, p: [((i, i + 1), OG.CNOT) for i in range(3)])
+    gates2 = [seq if g.name == "CHAIN" else g for g in gates]
+    psi = S.run(gates2, rng_draws(12), n)
+    assert abs(zz01(psi, n) - QG_ENERGY) > 1e-2
+
+
+def test_qg_eqd_golden():
+    n, gates = C.qg_test_circuit()
+    th = QG_ROUNDED_ANGLES
+    psi = S.run(gates, th, n)
+    F = M.qfim(S.derivative_states(gates, th, n), psi)
+    assert M.effective_quantum_dimension(F, 1e-12) == 12
+
+
+@pytest.mark.parametrize("n", [4, 6, 8])
+def test_npqc_qfim_identity(n):
+    """tests.py:103-128: QFIM == identity at the reference angles."""
+    for p in range(1, 2 ** (n // 2) + 1):
+        if n == 8 and p not in (1, 2, 5, 16):
+            continue  # keep the CPU suite short; all P run at n = 4, 6
+        gates, th = C.npqc(p, n)
+        psi = S.run(gates, th, n)
+        F = M.qfim(S.derivative_states(gates, th, n), psi)
+        assert np.max(np.abs(F - np.eye(len(th)))) < 1e-12
+
+
+def test_basic_gates():
+    """tests.py:64-87."""
+    psi = S.run([C.fixed_R_y(0, 1, np.pi / 2)], [], 1)
+    assert abs(psi[1].real - 1 / np.sqrt(2)) < 1e-15
+    psi = S.run([C.H(0, 1), C.H(0, 1)], [], 1)
+    assert np.allclose(psi, [1, 0], atol=1e-15)
+
+
+def test_textbook_identities():
+    assert np.allclose(OG.hadamard(), np.array([[1, 1], [1, -1]]) / np.sqrt(2))
+    assert np.allclose(OG.s_gate() @ OG.s_gate(), OG.SZ)
+    assert np.allclose(OG.t_gate() @ OG.t_gate(), OG.s_gate())
+    for f in (OG.rx, OG.ry, OG.rz):
+        u = f(0.37)
+        assert np.allclose(u.conj().T @ u, np.eye(2))
+    for p in "xyz":
+        u = OG.rpp(1.1, p)
+        assert np.allclose(u.conj().T @ u, np.eye(4))
+    # R_zz is diagonal with phases e^{-it/2} on even parity, e^{+it/2} on odd
+    t = 0.8
+    assert np.allclose(OG.rpp(t, "z"), np.diag(np.exp(-0.5j * t * np.array([1, -1, -1, 1]))))
+    # rotations are exp(-i t P / 2)
+    import scipy.linalg
+    for a in "xyz":
+        assert np.allclose(OG.ROT[a](0.9), scipy.linalg.expm(-0.45j * OG.PAULI[a]))
+
+
+def _random_circuit(n, rng):
+    gates = []
+    for _ in range(3):
+        for q in range(n):
+            gates.append([C.R_x, C.R_y, C.R_z][rng.integers(3)](q, n))
+        a, b = rng.choice(n, 2, replace=False)
+        gates.append([C.CNOT, C.CZ, C.sqrtiSWAP][rng.integers(3)]((int(a), int(b)), n))
+        a, b = rng.choice(n, 2, replace=False)
+        gates.append([C.R_zz, C.R_xx, C.R_yy][rng.integers(3)]((int(a), int(b)), n))
+        gates.append(C.fSim((int(b), int(a)), n))
+        gates += [C.H(rng.integers(n), n), C.T(rng.integers(n), n), C.S(rng.integers(n), n)]
+        gates.append(C.negative_R_z(rng.integers(n), n))
+        gates.append(C.offset_R_z(rng.integers(n), n, 0.3))
+    gates.append(C.CHAIN(C.CNOT, n))
+    gates.append(C.shared_parameter([C.R_x(q, n) for q in range(n)], n))
+    return gates
+
+
+@pytest.mark.parametrize("n", [2, 3, 5])
+def test_einsum_sim_matches_bruteforce(n):
+    rng = np.random.default_rng(7 + n)
+    gates = _random_circuit(n, rng)
+    th = rng.random(S.n_params(gates)) * 2 * np.pi
+    psi = S.run(gates, th, n)
+    u = S.dense_unitary(gates, th, n)
+    assert np.allclose(u.conj().T @ u, np.eye(2 ** n), atol=1e-12)
+    assert np.max(np.abs(psi - u[:, 0])) < 1e-13
+
+
+def test_embed_matches_kron_adjacent():
+    m = OG.SQRT_ISWAP @ OG.CNOT
+    assert np.allclose(S.embed(m, (1, 2), 4), np.kron(np.kron(np.eye(2), m), np.eye(2)))
+    # reversed order = swap-conjugated matrix
+    sw = np.eye(4)[[0, 2, 1, 3]]
+    assert np.allclose(S.embed(m, (2, 1), 4), np.kron(np.kron(np.eye(2), sw @ m @ sw), np.eye(2)))
+
+
+@pytest.mark.parametrize("maker", ["hea", "tfim", "npqc"])
+def test_derivative_states_match_central_fd(maker):
+    """For gates whose D satisfies D U = dU/dtheta the reference's derivative
+    states are the true partial derivatives; central FD at h = 1e-6."""
+    rng = np.random.default_rng(3)
+    init = None
+    if maker == "hea":
+        n, gates = 4, C.hea_linear(2, 4)
+    elif maker == "tfim":
+        n = 5
+        gates, init = C.tfim_open(2, n)
+    else:
+        n = 4
+        gates, _ = C.npqc(3, n)
+    th = rng.random(S.n_params(gates)) * 2 * np.pi
+    ds = S.derivative_states(gates, th, n, init)
+    h = 1e-6
+    for p in range(len(th)):
+        tp, tm = th.copy(), th.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (S.run(gates, tp, n, init) - S.run(gates, tm, n, init)) / (2 * h)
+        assert np.max(np.abs(fd - ds[p])) < 1e-8
+
+
+def test_single_ry_qfim_is_one():
+    gates = [C.R_y(0, 1)]
+    for t in (0.1, 1.3, 2.9):
+        psi = S.run(gates, [t], 1)
+        F = M.qfim(S.derivative_states(gates, [t], 1), psi)
+        assert abs(F[0, 0] - 1) < 1e-14
+
+
+def test_mw_bell_and_product():
+    bell11 = np.array([0, 1, -1, 0]) / np.sqrt(2)  # qutip bell_state('11')
+    assert abs(M.single_Q(bell11, 2) - 1) < 1e-14  # tests.py:289-295
+    prod = S.product_state(np.array([np.cos(0.3), np.exp(0.2j) * np.sin(0.3)]), 5)
+    assert abs(M.single_Q(prod, 5)) < 1e-14
+    ghz = np.zeros(8)
+    ghz[0] = ghz[7] = 1 / np.sqrt(2)
+    assert abs(M.single_Q(ghz, 3) - 1) < 1e-14
+
+
+def test_reduced_matches_partial_trace_definition():
+    rng = np.random.default_rng(5)
+    n = 4
+    psi = rng.normal(size=16) + 1j * rng.normal(size=16)
+    psi /= np.linalg.norm(psi)
+    rho = np.outer(psi, psi.conj()).reshape([2] * 8)
+    for k in range(n):
+        # trace every axis but k by explicit einsum subscripts
+        letters = "abcdefgh"
+        row = list(letters[:n])
+        col = list(letters[n:])
+        for j in range(n):
+            if j != k:
+                col[j] = row[j]
+        sub = "".join(row) + "".join(col) + "->" + row[k] + col[k]
+        ref = np.einsum(sub, rho)
+        assert np.allclose(M.reduced_1q(psi, k, n), ref, atol=1e-14)
+
+
+def test_expr_idle_closed_form():
+    """All F == 1 (idle circuit): all mass in the last bin, so
+    KL = -log(P_haar[last])."""
+    F = [1.0] * 300
+    dim = 2
+    bins = int(0.0075 * 300)
+    _, mids = M.gen_histo(F)
+    haar = (dim - 1) * (1 - mids) ** (dim - 2)
+    ph = haar / haar.sum()
+    assert len(mids) == bins
+    assert abs(M.expr(F, dim) - (-np.log(ph[-1]))) < 1e-12
+
+
+def test_expr_haar_samples_small():
+    """Haar states in dimension 4: KL against the analytic density is small."""
+    rng = np.random.default_rng(11)
+    d = 4
+    v = rng.normal(size=(2 * 4000, d)) + 1j * rng.normal(size=(2 * 4000, d))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    F = np.abs(np.sum(v[0::2].conj() * v[1::2], axis=1)) ** 2
+    assert M.expr(list(F), d, bins=30) < 0.02
+    assert M.expr(list(np.ones(4000)), d, bins=30) > 1.0
+
+
+def test_noncommuting_shared_reduces_for_diagonal_block():
+    """For a diagonal block conj(U) = U^dagger, so the commute=False derivative
+    of gates.py:458-466 equals the commute=True one."""
+    n = 3
+    layer = [C.R_zz((0, 1), n), C.R_z(2, n), C.R_zz((1, 2), n)]
+    a = C.shared_parameter(layer, n, commute=True)
+    b = C.shared_parameter(layer, n, commute=False)
+    psi = S.product_state(np.array([1, 1]) / np.sqrt(2), n)
+    t = 0.77
+    da = S.apply_deriv(S.apply_prims(psi, a.prims(t), n), a.deriv(t), n)
+    db = S.apply_deriv(S.apply_prims(psi, b.prims(t), n), b.deriv(t), n)
+    assert np.max(np.abs(da - db)) < 1e-14

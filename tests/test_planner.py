@@ -1,0 +1,206 @@
+"""CPU tests of the planner (libpqs host code, no GPU) and of the compiled op
+lists of the drop-in API against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import circuits as OC
+from oracle import gates as OG
+from oracle import sim as OS
+from tests.helpers import build_oracle, build_pkg, random_spec
+
+L = pytest.importorskip("pyramaterised_amd._lib")
+
+
+@pytest.fixture(scope="module")
+def pyqc():
+    import pyramaterised_amd
+    return pyramaterised_amd
+
+
+def op_matrix(o, params):
+    """(qubits, local matrix) of one pqs_op, restated from include/pqs.h."""
+    ang = o.offset + (o.scale * params[o.slot] if o.slot >= 0 else 0.0)
+    k = o.kind
+    m = np.array([complex(o.m[2 * i], o.m[2 * i + 1]) for i in range(16)])
+    if k == L.OP_MAT1:
+        return (o.q0,), m[:4].reshape(2, 2)
+    if k in (L.OP_RX, L.OP_RY, L.OP_RZ):
+        return (o.q0,), {L.OP_RX: OG.rx, L.OP_RY: OG.ry, L.OP_RZ: OG.rz}[k](ang)
+    if k == L.OP_MAT2:
+        return (o.q0, o.q1), m.reshape(4, 4)
+    if k == L.OP_CX:
+        return (o.q0, o.q1), OG.CNOT
+    if k == L.OP_CZ:
+        return (o.q0, o.q1), OG.CZ
+    if k in (L.OP_RXX, L.OP_RYY, L.OP_RZZ):
+        return (o.q0, o.q1), OG.rpp(ang, {L.OP_RXX: "x", L.OP_RYY: "y", L.OP_RZZ: "z"}[k])
+    if k == L.OP_FSIM:
+        ph = params[o.slot2] if o.slot2 >= 0 else 0.0
+        return (o.q0, o.q1), OG.fsim(ang, ph)
+    raise ValueError(k)
+
+
+def simulate_ops(ops, order, n, params, init=None):
+    psi = OS.zero_state(n) if init is None else init.copy()
+    for i in order:
+        qs, m = op_matrix(ops[i], params)
+        psi = OS.apply(psi, qs, m, n)
+    return psi
+
+
+def active_qubits(o):
+    k = o.kind
+    if k in (L.OP_RZ, L.OP_CZ, L.OP_RZZ):
+        return set()
+    if k == L.OP_MAT1:
+        return set() if (o.m[2] == 0 and o.m[3] == 0 and o.m[4] == 0 and o.m[5] == 0) else {o.q0}
+    if k == L.OP_CX:
+        return {o.q1}
+    if k in (L.OP_RX, L.OP_RY):
+        return {o.q0}
+    return {o.q0, o.q1}
+
+
+def test_library_loads_and_exports_every_header_symbol():
+    import os
+    import re
+    lib = L.lib()
+    assert lib.pqs_abi_version() == 1
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "pqs.h")).read()
+    names = set(re.findall(r"\b(pqs_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) >= 20
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    assert set(L.SIGNATURES) == names
+
+
+def test_plan_rejects_bad_ops(pyqc):
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd.gates import _op
+    with pytest.raises(L.EngineError):
+        E.Plan(3, [_op(L.OP_RX, 3, slot=0)], tile_bits=3)
+    with pytest.raises(L.EngineError):
+        E.Plan(3, [_op(L.OP_CX, 1, 1)], tile_bits=3)
+    with pytest.raises(L.EngineError):
+        E.Plan(3, [_op(L.OP_RX, 0, slot=0)], tile_bits=14)
+    # a generator acting on more qubits than the tile cannot be scheduled
+    from pyramaterised_amd.gates import _gen
+    ops = [_gen(0, 1, [q], -0.5j) for q in range(6)]
+    with pytest.raises(L.EngineError):
+        E.Plan(6, ops, tile_bits=4, low_bits=2)
+
+
+@pytest.mark.parametrize("n,tile,low,seed", [(6, 3, 2, 0), (8, 5, 3, 1), (10, 4, 1, 2), (12, 6, 3, 3),
+                                              (16, 13, 5, 4)])
+def test_sweep_order_is_equivalent_to_program_order(pyqc, n, tile, low, seed):
+    from pyramaterised_amd import engine as E
+    rng = np.random.default_rng(seed)
+    spec = [s for s in random_spec(n, 3, rng) if s[0] != "shared_Rx"]
+    qc = pyqc.PQC(n, tile_bits=tile, low_bits=low)
+    qc.add_layer(build_pkg(spec, n, pyqc))
+    ops = qc.ops()
+    plan = E.Plan(n, ops, tile_bits=tile, low_bits=low)
+    sweeps = plan.sweeps()
+    order = [i for _, idx in sweeps for i in idx]
+    assert sorted(order) == list(range(len(ops)))
+    for tq, idx in sweeps:
+        assert len(tq) == min(tile, n)
+        for i in idx:
+            assert active_qubits(ops[i]) <= set(tq)
+    params = rng.random(qc.n_true_params) * 2 * np.pi
+    a = simulate_ops(ops, range(len(ops)), n, params)
+    b = simulate_ops(ops, order, n, params)
+    assert np.max(np.abs(a - b)) < 1e-12
+    ref = OS.run(build_oracle(spec, n), params, n)
+    assert np.max(np.abs(a - ref)) < 1e-12
+
+
+def test_brickwork_28q_fuses_two_sweeps_per_layer(pyqc):
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    qc = pyqc.PQC(28, tile_bits=13, low_bits=5)
+    for l in T.ry_cz_brickwork_layers(30, 28):
+        qc.add_layer(l)
+    plan = E.Plan(28, qc.ops(), tile_bits=13, low_bits=5)
+    assert len(qc.ops()) == 1245
+    assert plan.n_sweeps <= 62
+
+
+def test_compiled_ops_match_oracle_templates(pyqc):
+    """The package's templates compile to the oracle's gate sequence."""
+    from pyramaterised_amd import templates as T
+    cases = [
+        (T.clifford_HE_layers(3, 5), OC.clifford_he(3, 5), 5, None),
+        (T.hea_linear_layers(3, 4), OC.hea_linear(3, 4), 4, None),
+        (T.generic_HE_layers(2, 6), OC.generic_he(2, 6), 6, None),
+        (T.TFIM_layers(2, 4), OC.tfim(2, 4), 4, None),
+        (T.ry_cz_brickwork_layers(4, 7), OC.ry_cz_brickwork(4, 7), 7, None),
+        (T.NPQC_layers(4, 6)[0], OC.npqc(4, 6)[0], 6, None),
+    ]
+    qg_p = pyqc.PQC(4)
+    for l in T.qg_circuit_layers(1, 4):
+        qg_p.add_layer(l)
+    rng = np.random.default_rng(0)
+    for layers, og, n, init in cases:
+        qc = pyqc.PQC(n)
+        for l in layers:
+            qc.add_layer(l)
+        assert qc.n_true_params == OS.n_params(og)
+        th = rng.random(qc.n_true_params) * 2 * np.pi
+        a = simulate_ops(qc.ops(), range(len(qc.ops())), n, th)
+        assert np.max(np.abs(a - OS.run(og, th, n))) < 1e-12
+
+
+def test_pqc_bookkeeping_quirks(pyqc):
+    """circuit.py:53-72: parameterised has a -1 after every gate, n_params is
+    twice the parameter count; set_params binds in gate order."""
+    n = 3
+    qc = pyqc.PQC(n)
+    qc.add_layer([pyqc.R_x(0, n), pyqc.H(1, n), pyqc.fSim([0, 2], n)])
+    assert qc.parameterised == [2, -1, -1, 5, 6, -1]
+    assert qc.n_params == 6
+    assert qc.n_true_params == 3
+    qc.set_params([0.1, 0.2, 0.3])
+    assert qc.get_params() == [0.1, 0.2, 0.3]
+    qc2 = pyqc.PQC(2)
+    qc2.add_layer([pyqc.negative_R_z(0, 2), pyqc.offset_R_z(1, 2, 0.5)])
+    qc2.set_params([0.4, 0.7])
+    # get_params returns the stored (transformed) angles like the reference
+    assert np.allclose(qc2.get_params(), [-0.4, 1.2])
+    assert np.allclose(qc2._gate_angles(), [0.4, 0.7])
+
+
+def test_random_params_equal_sequential_draws(pyqc):
+    from pyramaterised_amd import gates as G
+    from pyramaterised_amd import templates as T
+    qc = pyqc.PQC(3)
+    for l in T.clifford_HE_layers(2, 3):
+        qc.add_layer(l)
+    G.rng = np.random.default_rng(1)
+    batch = qc.random_params(4)
+    G.rng = np.random.default_rng(1)
+    seq = []
+    for _ in range(4):
+        qc.set_params("random")
+        seq.append(qc._angles.copy())
+    assert np.array_equal(batch, np.array(seq))
+
+
+def test_pauli_sum_algebra(pyqc):
+    from pyramaterised_amd import templates as T
+    H = T.TFIM_hamiltonian(4, g=0.6, h=0.2)
+    Z = np.diag([1, -1])
+    X = np.array([[0, 1], [1, 0]])
+
+    def emb(m, q, n):
+        return OS.kron_1q(m, q, n)
+    ref = np.zeros((16, 16), dtype=complex)
+    for i in range(4):
+        ref += emb(Z, i, 4) @ emb(Z, (i + 1) % 4, 4) + 0.6 * emb(X, i, 4) + 0.2 * emb(Z, i, 4)
+    assert np.allclose(H.full(), -ref)
+    Y = pyqc.PauliSum.single(2, 0, 2)
+    Xs = pyqc.PauliSum.single(2, 0, 1)
+    Zs = pyqc.PauliSum.single(2, 0, 3)
+    assert np.allclose((Xs * Y).full(), (1j * Zs).full())
+    dense = pyqc.PauliSum.from_dense(pyqc.Qobj(H.full()))
+    assert np.allclose(dense.full(), H.full())
