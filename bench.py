@@ -85,7 +85,9 @@ def headline(pyqc, args):
     S = 2 * n_pairs
     rng = np.random.default_rng(2024)
     params = torch.as_tensor(rng.random((S, P)) * 2 * np.pi, device="cuda")
-    plan = qc.plan()
+    t0 = time.perf_counter()
+    plan = qc.plan(batch=S)
+    jit_s = time.perf_counter() - t0
     init = qc.init()
     states = torch.empty((S, 2 ** qc.n_qubits), dtype=torch.complex128, device="cuda")
     m = pyqc.measure.Measurements(qc)
@@ -124,6 +126,7 @@ def headline(pyqc, args):
         "n_sweeps": plan.n_sweeps, "sweep_launch_ms": evolve_ms / launches,
         "bytes_per_launch": bytes_run / launches, "achieved_Bps": achieved, "expr_kl": float(expr),
         "mean_F": float(F.mean().item()), "counts_nonzero": int((counts > 0).sum()), "S": S, "P": P,
+        "jit": bool(getattr(plan, "jitted", False)), "jit_compile_s": jit_s,
     }
 
 
@@ -136,7 +139,7 @@ def leg_28q(pyqc, dtype, steps=3):
     for l in T.ry_cz_brickwork_layers(p, n):
         qc.add_layer(l)
     th = np.random.default_rng(7).random(qc.n_true_params) * 2 * np.pi
-    plan = qc.plan()
+    plan = qc.plan()  # specialised kernels (28 qubits)
     amp = 16 if dtype == "complex128" else 8
     out = torch.empty((1, 2 ** n), dtype=E.CDTYPE[plan.dtype], device="cuda")
     params = torch.as_tensor(th, device="cuda")
@@ -154,6 +157,7 @@ def leg_28q(pyqc, dtype, steps=3):
     del out
     torch.cuda.empty_cache()
     return {"dtype": dtype, "ms": ms, "gates_per_s": 1245 / (ms * 1e-3), "n_sweeps": plan.n_sweeps,
+            "jit": bool(getattr(plan, "jitted", False)),
             "GBps": b / (ms * 1e-3) / 1e9, "hbm_frac": b / (ms * 1e-3) / PEAK_HBM, "norm": norm}
 
 
@@ -199,6 +203,7 @@ def speedups_16q(pyqc, args, head):
     import torch
     from pyramaterised_amd import engine as E
     qc = build_c2(pyqc)
+    qc.plan(batch=64)  # same specialised kernels for the plain path
     P = qc.n_true_params
     rng = np.random.default_rng(11)
     # plain expressibility: states one at a time, overlaps one at a time

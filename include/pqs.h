@@ -212,6 +212,24 @@ int pqs_abs2(pqs_ctx *ctx, const double *z_dev, int64_t n, double *out_dev, void
 int pqs_magic(pqs_ctx *ctx, int32_t dtype, const void *states_dev, int32_t n_qubits,
               int64_t batch, double alpha, double *out_dev, void *stream);
 
+/* Run-time specialisation (hiprtc) of the plan's sweeps: every sweep made of
+ * one-qubit rotations, fixed one-qubit matrices, CNOT, CZ and R_zz gets its
+ * own straight-line kernel with register and bit indices as constants; other
+ * sweeps stay interpreted.  load = 0 compiles only (no device needed, used
+ * to check generated code); load = 1 also loads the module on ctx's device,
+ * after which pqs_run / pqs_run_batch / pqs_jacobian launch the specialised
+ * kernels.  No reference counterpart (the reference has no kernels). */
+int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *plan, int32_t load);
+/* The generated source (for inspection); *len gets its length. */
+int pqs_plan_jit_source(const pqs_plan *plan, char *buf, int64_t cap, int64_t *len,
+                        int32_t *n_jit_sweeps);
+
+/* Source of sweep s's specialised kernel (*len = 0 if it is interpreted). */
+int pqs_plan_jit_sweep_source(const pqs_plan *plan, int32_t s, char *buf, int64_t cap, int64_t *len);
+/* Compile one kernel source into the code-object cache (no device needed;
+ * lets several processes share the compile work of pqs_plan_jit). */
+int pqs_jit_prepare(const char *src, int64_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,10 @@ SIGNATURES = {
     "pqs_histogram": (C.c_int, [vp, vp, i64, vp, i32, vp, vp]),
     "pqs_abs2": (C.c_int, [vp, vp, i64, vp, vp]),
     "pqs_magic": (C.c_int, [vp, i32, vp, i32, i64, C.c_double, vp, vp]),
+    "pqs_plan_jit": (C.c_int, [vp, vp, i32]),
+    "pqs_plan_jit_source": (C.c_int, [vp, C.c_char_p, i64, C.POINTER(i64), C.POINTER(i32)]),
+    "pqs_plan_jit_sweep_source": (C.c_int, [vp, i32, C.c_char_p, i64, C.POINTER(i64)]),
+    "pqs_jit_prepare": (C.c_int, [C.c_char_p, i64]),
 }
 
 _lock = threading.Lock()

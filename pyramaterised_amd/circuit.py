@@ -24,7 +24,7 @@ from .qobj import Ket, PauliSum, ProductKet, Qobj, basis, expect, tensor
 class PQC:
     """An n-qubit, p-layer parameterised quantum circuit (circuit.py:12-198)."""
 
-    def __init__(self, n_qubits, dtype="complex128", tile_bits=0, low_bits=0, block=0):
+    def __init__(self, n_qubits, dtype="complex128", tile_bits=0, low_bits=0, block=0, jit="auto"):
         self.n_qubits = n_qubits
         self.n_layers = 0
         self.layers = []
@@ -37,6 +37,10 @@ class PQC:
         self.state = self.initial_state
         self.dtype = E.dtype_code(dtype)
         self.plan_opts = dict(tile_bits=tile_bits, low_bits=low_bits, block=block)
+        # run-time specialised sweep kernels: "auto" uses them for batches and
+        # large states, where their one-off compile pays back; PQS_JIT=0/1
+        # overrides
+        self.jit = jit
         self._angles = None
         self._plans = {}
         self.set_gates()
@@ -163,14 +167,25 @@ class PQC:
                     out += g.gen_ops(slot + w, slot, which=w + 1)
         return out
 
-    def plan(self, with_generators=False, dtype=None):
+    def plan(self, with_generators=False, dtype=None, batch=1):
         dt = self.dtype if dtype is None else E.dtype_code(dtype)
         key = (with_generators, dt)
         p = self._plans.get(key)
         if p is None:
             p = E.Plan(self.n_qubits, self.ops(with_generators), dtype=dt, **self.plan_opts)
             self._plans[key] = p
+        if not getattr(p, "jitted", False) and self._want_jit(batch):
+            p.jit()
         return p
+
+    def _want_jit(self, batch):
+        import os
+        env = os.environ.get("PQS_JIT")
+        if env is not None:
+            return env not in ("0", "", "false", "False")
+        if self.jit == "auto":
+            return batch >= 32 or self.n_qubits >= 20
+        return bool(self.jit)
 
     def init(self):
         st = self.initial_state
@@ -214,7 +229,7 @@ class PQC:
             return torch.empty((0, 2 ** self.n_qubits), dtype=E.CDTYPE[self.dtype], device=E.device())
         last = params[-1].tolist() if not torch.is_tensor(params) else params[-1].cpu().tolist()
         self.set_params(last)
-        return E.run(self.plan(), params, self.init())
+        return E.run(self.plan(batch=len(params)), params, self.init())
 
     def update_state(self, angles):
         """circuit.py:127-130."""
@@ -262,7 +277,7 @@ class PQC:
 
     def jacobian(self, method="forward"):
         """(psi, J) with J[t] = d psi / d theta_t as tensors."""
-        plan = self.plan(with_generators=True)
+        plan = self.plan(with_generators=True, batch=self.n_true_params)
         if method == "forward":
             return E.jacobian(plan, self._params_vec(), self.init())
         if method == "plain":

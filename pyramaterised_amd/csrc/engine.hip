@@ -17,7 +17,14 @@
 // else runs in the same batch.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -34,6 +41,7 @@ namespace {
 using pqs::DevOp;
 using pqs::DevSweep;
 using pqs::DevTerm;
+using pqs::DevPhase;
 
 int fail(int code, const std::string &msg) {
   pqs::set_error(msg);
@@ -182,6 +190,135 @@ __device__ __forceinline__ typename CT<R>::V cvt(double2 a) {
   return typename CT<R>::V{R(a.x), R(a.y)};
 }
 
+__device__ __forceinline__ int swz(int j) { return j ^ (((j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15); }
+
+template <typename R, int MAXPER, bool SWZ>
+__device__ __forceinline__ void apply_lds_op(typename CT<R>::V *tile, const int S, const DevOp *op,
+                                             const uint64_t base, const double *params,
+                                             const DevTerm *terms, const int tid, const int nthr) {
+  using V = typename CT<R>::V;
+  const int kind = op->kind;
+  const int t0 = op->t0, t1 = op->t1;
+  auto IX = [](int j) { return SWZ ? swz(j) : j; };
+  switch (kind) {
+      case PQS_OP_MAT1:
+      case PQS_OP_RX:
+      case PQS_OP_RY:
+      case PQS_OP_RZ: {
+        M2 M;
+        mat1_of(op, params, M);
+        if (t0 < 0) {  // diagonal op on a bit constant over the tile
+          const V d = cvt<R>(M.m[((base >> op->g0) & 1) ? 3 : 0]);
+          for (int j = tid; j < S; j += nthr) tile[IX(j)] = cmul(tile[IX(j)], d);
+        } else if (kind == PQS_OP_RZ) {
+          const V d0 = cvt<R>(M.m[0]), d1 = cvt<R>(M.m[3]);
+          for (int j = tid; j < S; j += nthr) tile[IX(j)] = cmul(tile[IX(j)], ((j >> t0) & 1) ? d1 : d0);
+        } else {
+          const V m00 = cvt<R>(M.m[0]), m01 = cvt<R>(M.m[1]), m10 = cvt<R>(M.m[2]), m11 = cvt<R>(M.m[3]);
+          for (int u = tid; u < (S >> 1); u += nthr) {
+            const int j0 = ins0(u, t0), j1 = j0 | (1 << t0);
+            const V a = tile[IX(j0)], b = tile[IX(j1)];
+            tile[IX(j0)] = cadd(cmul(m00, a), cmul(m01, b));
+            tile[IX(j1)] = cadd(cmul(m10, a), cmul(m11, b));
+          }
+        }
+        break;
+      }
+      case PQS_OP_CX: {
+        const bool ctl_in = t0 >= 0;
+        const bool ctl_on = !ctl_in && ((base >> op->g0) & 1);
+        if (ctl_in || ctl_on) {
+          for (int u = tid; u < (S >> 1); u += nthr) {
+            const int j0 = ins0(u, t1);
+            if (ctl_in && !((j0 >> t0) & 1)) continue;
+            const int j1 = j0 | (1 << t1);
+            const V a = tile[IX(j0)];
+            tile[IX(j0)] = tile[IX(j1)];
+            tile[IX(j1)] = a;
+          }
+        }
+        break;
+      }
+      case PQS_OP_CZ:
+      case PQS_OP_RZZ: {
+        V d[4];
+        if (kind == PQS_OP_CZ) {
+          d[0] = d[1] = d[2] = V{R(1), R(0)};
+          d[3] = V{R(-1), R(0)};
+        } else {
+          double s_, c_;
+          sincos(0.5 * op_angle(op, params), &s_, &c_);
+          d[0] = d[3] = V{R(c_), R(-s_)};
+          d[1] = d[2] = V{R(c_), R(s_)};
+        }
+        const int c0 = t0 < 0 ? int((base >> op->g0) & 1) : -1;
+        const int c1 = t1 < 0 ? int((base >> op->g1) & 1) : -1;
+        for (int j = tid; j < S; j += nthr) {
+          const int b0 = c0 >= 0 ? c0 : ((j >> t0) & 1);
+          const int b1 = c1 >= 0 ? c1 : ((j >> t1) & 1);
+          tile[IX(j)] = cmul(tile[IX(j)], d[2 * b0 + b1]);
+        }
+        break;
+      }
+      case PQS_OP_MAT2:
+      case PQS_OP_RXX:
+      case PQS_OP_RYY:
+      case PQS_OP_FSIM:
+      case PQS_OP_GEN_FSIM: {
+        M4 M;
+        mat2_of(op, params, M);
+        V m[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) m[i] = cvt<R>(M.m[i]);
+        const int lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+        for (int u = tid; u < (S >> 2); u += nthr) {
+          const int j = ins0(ins0(u, lo), hi);
+          const int jj[4] = {j, j | (1 << t1), j | (1 << t0), j | (1 << t0) | (1 << t1)};
+          V a[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) a[c] = tile[IX(jj[c])];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            V acc = cmul(m[4 * r], a[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(m[4 * r + c], a[c]));
+            tile[IX(jj[r])] = acc;
+          }
+        }
+        break;
+      }
+      case pqs::kPauliSum: {
+                V acc[MAXPER];
+        const int nt = op->nterms;
+        const DevTerm *tm = terms + op->term0;
+#pragma unroll
+        for (int m = 0; m < MAXPER; ++m) {
+          const int j = tid + m * nthr;
+          V a{R(0), R(0)};
+          if (j < S) {
+            for (int t = 0; t < nt; ++t) {
+              const int jj = j ^ int(tm[t].xm);
+              const int par = __popc(uint32_t(jj) & tm[t].zm_tile) + __popcll(base & tm[t].zm_glob);
+              V p = cmul(V{R(tm[t].cre), R(tm[t].cim)}, tile[IX(jj)]);
+              if (par & 1) p = V{-p.x, -p.y};
+              a = cadd(a, p);
+            }
+          }
+          acc[m] = a;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < MAXPER; ++m) {
+          const int j = tid + m * nthr;
+          if (j < S) tile[IX(j)] = acc[m];
+        }
+        break;
+      }
+    default:
+      break;
+  }
+}
+
 template <typename R, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sweep(const DevSweep sw, const DevOp *__restrict__ ops,
                                                  const DevTerm *__restrict__ terms,
@@ -247,127 +384,8 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const DevSweep sw, const DevOp 
     const DevOp *op = ops + o;
     const int tg = op->tangent;
     if (tg >= 0 && tg != in.tangent) continue;
-    const int kind = op->kind;
-    const int t0 = op->t0, t1 = op->t1;
     __syncthreads();
-    switch (kind) {
-      case PQS_OP_MAT1:
-      case PQS_OP_RX:
-      case PQS_OP_RY:
-      case PQS_OP_RZ: {
-        M2 M;
-        mat1_of(op, params, M);
-        if (t0 < 0) {  // diagonal op on a bit constant over the tile
-          const V d = cvt<R>(M.m[((base >> op->g0) & 1) ? 3 : 0]);
-          for (int j = tid; j < S; j += BLOCK) tile[j] = cmul(tile[j], d);
-        } else if (kind == PQS_OP_RZ) {
-          const V d0 = cvt<R>(M.m[0]), d1 = cvt<R>(M.m[3]);
-          for (int j = tid; j < S; j += BLOCK) tile[j] = cmul(tile[j], ((j >> t0) & 1) ? d1 : d0);
-        } else {
-          const V m00 = cvt<R>(M.m[0]), m01 = cvt<R>(M.m[1]), m10 = cvt<R>(M.m[2]), m11 = cvt<R>(M.m[3]);
-          for (int u = tid; u < (S >> 1); u += BLOCK) {
-            const int j0 = ins0(u, t0), j1 = j0 | (1 << t0);
-            const V a = tile[j0], b = tile[j1];
-            tile[j0] = cadd(cmul(m00, a), cmul(m01, b));
-            tile[j1] = cadd(cmul(m10, a), cmul(m11, b));
-          }
-        }
-        break;
-      }
-      case PQS_OP_CX: {
-        const bool ctl_in = t0 >= 0;
-        const bool ctl_on = !ctl_in && ((base >> op->g0) & 1);
-        if (ctl_in || ctl_on) {
-          for (int u = tid; u < (S >> 1); u += BLOCK) {
-            const int j0 = ins0(u, t1);
-            if (ctl_in && !((j0 >> t0) & 1)) continue;
-            const int j1 = j0 | (1 << t1);
-            const V a = tile[j0];
-            tile[j0] = tile[j1];
-            tile[j1] = a;
-          }
-        }
-        break;
-      }
-      case PQS_OP_CZ:
-      case PQS_OP_RZZ: {
-        V d[4];
-        if (kind == PQS_OP_CZ) {
-          d[0] = d[1] = d[2] = V{R(1), R(0)};
-          d[3] = V{R(-1), R(0)};
-        } else {
-          double s_, c_;
-          sincos(0.5 * op_angle(op, params), &s_, &c_);
-          d[0] = d[3] = V{R(c_), R(-s_)};
-          d[1] = d[2] = V{R(c_), R(s_)};
-        }
-        const int c0 = t0 < 0 ? int((base >> op->g0) & 1) : -1;
-        const int c1 = t1 < 0 ? int((base >> op->g1) & 1) : -1;
-        for (int j = tid; j < S; j += BLOCK) {
-          const int b0 = c0 >= 0 ? c0 : ((j >> t0) & 1);
-          const int b1 = c1 >= 0 ? c1 : ((j >> t1) & 1);
-          tile[j] = cmul(tile[j], d[2 * b0 + b1]);
-        }
-        break;
-      }
-      case PQS_OP_MAT2:
-      case PQS_OP_RXX:
-      case PQS_OP_RYY:
-      case PQS_OP_FSIM:
-      case PQS_OP_GEN_FSIM: {
-        M4 M;
-        mat2_of(op, params, M);
-        V m[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) m[i] = cvt<R>(M.m[i]);
-        const int lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
-        for (int u = tid; u < (S >> 2); u += BLOCK) {
-          const int j = ins0(ins0(u, lo), hi);
-          const int jj[4] = {j, j | (1 << t1), j | (1 << t0), j | (1 << t0) | (1 << t1)};
-          V a[4];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) a[c] = tile[jj[c]];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            V acc = cmul(m[4 * r], a[0]);
-#pragma unroll
-            for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(m[4 * r + c], a[c]));
-            tile[jj[r]] = acc;
-          }
-        }
-        break;
-      }
-      case pqs::kPauliSum: {
-        constexpr int MAXPER = (1 << (sizeof(R) == 8 ? 13 : 14)) / BLOCK;
-        V acc[MAXPER];
-        const int nt = op->nterms;
-        const DevTerm *tm = terms + op->term0;
-#pragma unroll
-        for (int m = 0; m < MAXPER; ++m) {
-          const int j = tid + m * BLOCK;
-          V a{R(0), R(0)};
-          if (j < S) {
-            for (int t = 0; t < nt; ++t) {
-              const int jj = j ^ int(tm[t].xm);
-              const int par = __popc(uint32_t(jj) & tm[t].zm_tile) + __popcll(base & tm[t].zm_glob);
-              V p = cmul(V{R(tm[t].cre), R(tm[t].cim)}, tile[jj]);
-              if (par & 1) p = V{-p.x, -p.y};
-              a = cadd(a, p);
-            }
-          }
-          acc[m] = a;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < MAXPER; ++m) {
-          const int j = tid + m * BLOCK;
-          if (j < S) tile[j] = acc[m];
-        }
-        break;
-      }
-      default:
-        break;
-    }
+    apply_lds_op<R, (1 << (sizeof(R) == 8 ? 13 : 14)) / BLOCK, false>(tile, S, op, base, params, terms, tid, BLOCK);
   }
   __syncthreads();
 
@@ -378,6 +396,461 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const DevSweep sw, const DevOp 
       gi |= uint64_t((j >> sw.run_shift[r]) & ((1 << sw.run_len[r]) - 1)) << sw.run_pos[r];
     dst[gi] = tile[j];
   }
+}
+
+// ------------------------------------------------------- register executor
+// Tiles of >= 10 qubits: each thread holds NA = 16 amplitudes (kRegBits = 4
+// tile bits) in VGPRs; a 2^13 tile uses 512 threads (two waves per SIMD).
+// A sweep is a list of phases; a register phase reads the thread's 16
+// amplitudes from LDS for its register bits, applies all its ops in registers
+// (the register bit of every op is dispatched to a template instance, so
+// array indices stay compile-time constants), and writes them back.  LDS
+// traffic is one read + one write of the tile per phase instead of per op.
+// The sweep's ops are staged in LDS as compact records with their cos / sin
+// precomputed, and the LDS tile is XOR-swizzled so that every phase's access
+// pattern spreads over the banks.
+constexpr int NA = 1 << pqs::kRegBits;
+
+// Run a statement with a compile-time register bit B_ (or C_) equal to the
+// runtime value r (0..3), so that register arrays keep constant indices.
+#define SW4(r, ...)                                       \
+  switch (r) {                                            \
+    case 0: { constexpr int B_ = 0; __VA_ARGS__; } break; \
+    case 1: { constexpr int B_ = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int B_ = 2; __VA_ARGS__; } break; \
+    default: { constexpr int B_ = 3; __VA_ARGS__; } break; \
+  }
+#define SW4C(r, ...)                                      \
+  switch (r) {                                            \
+    case 0: { constexpr int C_ = 0; __VA_ARGS__; } break; \
+    case 1: { constexpr int C_ = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int C_ = 2; __VA_ARGS__; } break; \
+    default: { constexpr int C_ = 3; __VA_ARGS__; } break; \
+  }
+
+// Rotations in scaled form.  R_y(a) = c [[1, -t], [t, 1]] with t = s / c
+// (|c| >= |s|, form A) or s [[u, -1], [1, u]] with u = c / s (form B); the
+// scalar factor (c or s) is the same for every amplitude, so it is folded
+// into a per-tile running factor and the pair update costs one FMA per
+// output component.  R_x and the diagonal R_z / R_zz phases likewise.
+template <int B, bool FB, typename V, typename R>
+__device__ __forceinline__ void reg_ry(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if (v & (1 << B)) continue;
+    const V x = a[v], y = a[v | (1 << B)];
+    if constexpr (!FB) {  // [[1, -t], [t, 1]]
+      a[v] = V{fma(-t, y.x, x.x), fma(-t, y.y, x.y)};
+      a[v | (1 << B)] = V{fma(t, x.x, y.x), fma(t, x.y, y.y)};
+    } else {  // [[u, -1], [1, u]]
+      a[v] = V{fma(t, x.x, -y.x), fma(t, x.y, -y.y)};
+      a[v | (1 << B)] = V{fma(t, y.x, x.x), fma(t, y.y, x.y)};
+    }
+  }
+}
+
+template <int B, bool FB, typename V, typename R>
+__device__ __forceinline__ void reg_rx(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if (v & (1 << B)) continue;
+    const V x = a[v], y = a[v | (1 << B)];
+    if constexpr (!FB) {  // [[1, -i t], [-i t, 1]]
+      a[v] = V{fma(t, y.y, x.x), fma(-t, y.x, x.y)};
+      a[v | (1 << B)] = V{fma(t, x.y, y.x), fma(-t, x.x, y.y)};
+    } else {  // [[u, -i], [-i, u]]
+      a[v] = V{fma(t, x.x, y.y), fma(t, x.y, -y.x)};
+      a[v | (1 << B)] = V{fma(t, y.x, x.y), fma(t, y.y, -x.x)};
+    }
+  }
+}
+
+// x * (1 - i t) (form A) or x * (u - i) (form B); conj for the other sign
+template <bool FB, bool NEG, typename V, typename R>
+__device__ __forceinline__ V phase_mul(V x, R t) {
+  if constexpr (!FB) {
+    return NEG ? V{fma(t, x.y, x.x), fma(-t, x.x, x.y)} : V{fma(-t, x.y, x.x), fma(t, x.x, x.y)};
+  } else {
+    return NEG ? V{fma(t, x.x, x.y), fma(t, x.y, -x.x)} : V{fma(t, x.x, -x.y), fma(t, x.y, x.x)};
+  }
+}
+
+// diag(e^{-ia/2}, e^{+ia/2}) on register bit B (scaled)
+template <int B, bool FB, typename V, typename R>
+__device__ __forceinline__ void reg_rz(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = (v & (1 << B)) ? phase_mul<FB, false>(a[v], t) : phase_mul<FB, true>(a[v], t);
+}
+
+// e^{-ia/2} on even parity of register bits (B0, B1), e^{+ia/2} on odd (scaled)
+template <int B0, int B1, bool FB, typename V, typename R>
+__device__ __forceinline__ void reg_rzz(V (&a)[NA], R t) {
+  if constexpr (B0 != B1) {
+#pragma unroll
+    for (int v = 0; v < NA; ++v)
+      a[v] = (((v >> B0) ^ (v >> B1)) & 1) ? phase_mul<FB, false>(a[v], t) : phase_mul<FB, true>(a[v], t);
+  }
+}
+
+// the same phase (sign NEG) on every amplitude (scaled)
+template <bool FB, bool NEG, typename V, typename R>
+__device__ __forceinline__ void reg_phase_all(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = phase_mul<FB, NEG>(a[v], t);
+}
+
+template <int B, typename V>
+__device__ __forceinline__ void reg_mat1(V (&a)[NA], V m00, V m01, V m10, V m11) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if (v & (1 << B)) continue;
+    const V x = a[v], y = a[v | (1 << B)];
+    a[v] = cadd(cmul(m00, x), cmul(m01, y));
+    a[v | (1 << B)] = cadd(cmul(m10, x), cmul(m11, y));
+  }
+}
+
+// multiply amplitudes with register bit B clear by d0, set by d1
+template <int B, typename V>
+__device__ __forceinline__ void reg_diag(V (&a)[NA], V d0, V d1) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = cmul(a[v], (v & (1 << B)) ? d1 : d0);
+}
+
+template <typename V>
+__device__ __forceinline__ void reg_scale(V (&a)[NA], V d) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = cmul(a[v], d);
+}
+
+template <typename V>
+__device__ __forceinline__ V cneg(V x) {
+  return V{-x.x, -x.y};
+}
+
+// CNOT with target register bit BT and control register bit BC
+template <int BT, int BC, typename V>
+__device__ __forceinline__ void reg_cx(V (&a)[NA]) {
+  if constexpr (BT != BC) {
+#pragma unroll
+    for (int v = 0; v < NA; ++v) {
+      if ((v & (1 << BT)) || !(v & (1 << BC))) continue;
+      const V x = a[v];
+      a[v] = a[v | (1 << BT)];
+      a[v | (1 << BT)] = x;
+    }
+  }
+}
+
+// X on register bit BT (a CNOT whose control is a set constant bit)
+template <int BT, typename V>
+__device__ __forceinline__ void reg_x(V (&a)[NA]) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if (v & (1 << BT)) continue;
+    const V x = a[v];
+    a[v] = a[v | (1 << BT)];
+    a[v | (1 << BT)] = x;
+  }
+}
+
+// -1 where register bits B0 and B1 are both set
+template <int B0, int B1, typename V>
+__device__ __forceinline__ void reg_cz(V (&a)[NA]) {
+  if constexpr (B0 != B1) {
+#pragma unroll
+    for (int v = 0; v < NA; ++v)
+      if ((v & (1 << B0)) && (v & (1 << B1))) a[v] = cneg(a[v]);
+  }
+}
+
+// -1 where register bit B is set
+template <int B, typename V>
+__device__ __forceinline__ void reg_z(V (&a)[NA]) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v)
+    if (v & (1 << B)) a[v] = cneg(a[v]);
+}
+
+// Compact op record staged in LDS (32 bytes).
+struct alignas(16) ROp {
+  int32_t kind;
+  int32_t regs;  // r0 | r1 << 8 (0xff: not a register bit)
+  int32_t t01;   // t0 | t1 << 8 (tile bit, 0xff: outside the tile)
+  int32_t g01;   // g0 | g1 << 8 (global bit position)
+  double t, f;   // rotations: ratio t (or u) and scalar factor; form B if regs bit 16
+};
+
+template <typename R, int MAXT, bool LDSOPS>
+__global__ __launch_bounds__(MAXT) void k_sweep_reg(const DevSweep sw, const DevOp *__restrict__ ops,
+                                                    const DevTerm *__restrict__ terms,
+                                                    const DevPhase *__restrict__ phases,
+                                                    const InstDev *__restrict__ insts, int64_t n_active,
+                                                    const typename CT<R>::V *__restrict__ init_tab, int n, int s,
+                                                    int n_sweeps) {
+  using V = typename CT<R>::V;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V *tile = reinterpret_cast<V *>(smem_raw);
+  const int S = 1 << sw.k;
+  ROp *rops = reinterpret_cast<ROp *>(smem_raw + size_t(S) * sizeof(V));
+  const int tid = threadIdx.x;
+  const int nthr = S >> pqs::kRegBits;
+  const int64_t gid = blockIdx.x;
+  const int64_t inst_idx = gid / sw.n_tiles;
+  const int64_t tile_id = gid - inst_idx * sw.n_tiles;
+  if (inst_idx >= n_active) return;
+  const InstDev in = insts[inst_idx];
+
+  uint64_t base = 0;
+  for (int r = 0; r < sw.cnruns; ++r)
+    base |= (uint64_t(tile_id >> sw.crun_shift[r]) & ((uint64_t(1) << sw.crun_len[r]) - 1))
+            << sw.crun_pos[r];
+
+  const V *src;
+  V *dst;
+  int init = 0;
+  if (in.pingpong) {
+    dst = ((n_sweeps - 1 - s) & 1) == 0 ? (V *)in.state : (V *)in.alt;
+    if (s == in.start_sweep) {
+      src = (const V *)in.src;
+      init = in.src ? 0 : in.init;
+    } else {
+      src = ((n_sweeps - s) & 1) == 0 ? (const V *)in.state : (const V *)in.alt;
+    }
+  } else {
+    dst = (V *)in.state;
+    if (s == in.start_sweep) {
+      src = in.src ? (const V *)in.src : dst;
+      init = in.src ? 0 : in.init;
+    } else {
+      src = dst;
+    }
+  }
+  const double *params = in.params;
+
+  // ---- gather the tile into LDS (coalesced; swizzled slots).  Tile index
+  // j = tid + m * nthr: the low k-4 bits come from tid, the top 4 from m, so
+  // the global index is base | dep(tid) | dep(m << (k-4)); all loads of a
+  // thread are issued before any is consumed.
+  const int kt = sw.k - pqs::kRegBits;
+  uint64_t dl = base;
+  uint64_t hp[pqs::kRegBits];
+  for (int r = 0; r < sw.nruns; ++r) {
+    for (int b = 0; b < sw.run_len[r]; ++b) {
+      const int i = sw.run_shift[r] + b;  // tile bit
+      const uint64_t gb = uint64_t(1) << (sw.run_pos[r] + b);
+      if (i < kt) {
+        if ((tid >> i) & 1) dl |= gb;
+      } else {
+        hp[i - kt] = gb;
+      }
+    }
+  }
+  auto gaddr = [&](int m) -> uint64_t {
+    uint64_t g = dl;
+#pragma unroll
+    for (int b = 0; b < pqs::kRegBits; ++b)
+      if (m & (1 << b)) g |= hp[b];
+    return g;
+  };
+  {
+    V tmp[NA];
+    if (init == 0) {
+#pragma unroll
+      for (int m = 0; m < NA; ++m) tmp[m] = src[gaddr(m)];
+    } else if (init == 1) {
+#pragma unroll
+      for (int m = 0; m < NA; ++m) tmp[m] = V{gaddr(m) == 0 ? R(1) : R(0), R(0)};
+    } else {
+#pragma unroll
+      for (int m = 0; m < NA; ++m) {
+        const uint64_t gi = gaddr(m);
+        V v{R(1), R(0)};
+        for (int q = 0; q < n; ++q) v = cmul(v, init_tab[2 * q + int((gi >> (n - 1 - q)) & 1)]);
+        tmp[m] = v;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NA; ++m) tile[swz(tid + m * nthr)] = tmp[m];
+  }
+  // ---- stage the sweep's op records (with cos / sin of half angles)
+  for (int o = sw.op_begin + tid; o < sw.op_end; o += nthr) {
+    const DevOp *d = ops + o;
+    ROp r;
+    r.kind = d->kind;
+    r.regs = (d->r0 & 0xff) | ((d->r1 & 0xff) << 8);
+    r.t01 = (d->t0 & 0xff) | ((d->t1 & 0xff) << 8);
+    r.g01 = (d->g0 & 0xff) | ((d->g1 & 0xff) << 8);
+    r.t = 0.0;
+    r.f = 1.0;
+    if (d->coef >= 0) {
+      double sn, cs;
+      sincos(0.5 * op_angle(d, params), &sn, &cs);
+      if (fabs(cs) >= fabs(sn)) {
+        r.t = sn / cs;
+        r.f = cs;
+      } else {
+        r.t = cs / sn;
+        r.f = sn;
+        r.regs |= 1 << 16;
+      }
+    }
+    rops[o - sw.op_begin] = r;
+  }
+  __syncthreads();
+
+  for (int ph = sw.phase_begin; ph < sw.phase_end; ++ph) {
+    const DevPhase *P = phases + ph;
+    if (P->lds) {
+      if constexpr (LDSOPS) {
+        for (int o = P->op_begin; o < P->op_end; ++o) {
+          const DevOp *op = ops + o;
+          if (op->tangent >= 0 && op->tangent != in.tangent) continue;
+          apply_lds_op<R, NA, true>(tile, S, op, base, params, terms, tid, nthr);
+          __syncthreads();
+        }
+      }
+      continue;
+    }
+    // thread part of the tile index and the swizzled register-bit offsets
+    int toff = 0;
+    for (int i = 0; i < P->ntb; ++i) toff |= ((tid >> i) & 1) << P->tb[i];
+    const int pt = swz(toff);
+    int sr[pqs::kRegBits];
+#pragma unroll
+    for (int i = 0; i < pqs::kRegBits; ++i) sr[i] = swz(1 << P->rb[i]);
+    auto laddr = [&](int v) {
+      int addr = pt;
+#pragma unroll
+      for (int i = 0; i < pqs::kRegBits; ++i)
+        if (v & (1 << i)) addr ^= sr[i];
+      return addr;
+    };
+    V a[NA];
+#pragma unroll
+    for (int v = 0; v < NA; ++v) a[v] = tile[laddr(v)];
+    double fscale = 1.0;
+    // constant bit of a qubit that is not a register bit
+    auto cbit = [&](int t, int g) -> int {
+      return t != 0xff ? ((toff >> t) & 1) : int((base >> g) & 1);
+    };
+    for (int o = P->op_begin; o < P->op_end; ++o) {
+      const ROp op = rops[o - sw.op_begin];
+      // the record is the same for every lane: keep its fields in SGPRs so
+      // that dispatch is scalar branching, not exec-masked case walking
+      const int kind = __builtin_amdgcn_readfirstlane(op.kind);
+      const int regs = __builtin_amdgcn_readfirstlane(op.regs);
+      const int t01 = __builtin_amdgcn_readfirstlane(op.t01);
+      const int g01 = __builtin_amdgcn_readfirstlane(op.g01);
+      const int r0 = regs & 0xff, r1 = (regs >> 8) & 0xff;
+      const int t0 = t01 & 0xff, t1 = (t01 >> 8) & 0xff;
+      const int g0 = g01 & 0xff, g1 = (g01 >> 8) & 0xff;
+      const R t = R(op.t);
+      const bool fb = (regs >> 16) & 1;
+      switch (kind) {
+        case PQS_OP_RY:
+          fscale *= op.f;
+          if (fb) {
+            SW4(r0, reg_ry<B_, true>(a, t));
+          } else {
+            SW4(r0, reg_ry<B_, false>(a, t));
+          }
+          break;
+        case PQS_OP_RX:
+          fscale *= op.f;
+          if (fb) {
+            SW4(r0, reg_rx<B_, true>(a, t));
+          } else {
+            SW4(r0, reg_rx<B_, false>(a, t));
+          }
+          break;
+        case PQS_OP_RZ:
+        case PQS_OP_RZZ: {
+          fscale *= op.f;
+          // register bits involved and the constant bits' parity
+          const bool q0r = r0 != 0xff;
+          const bool q1r = kind == PQS_OP_RZZ && r1 != 0xff;
+          int cpar = 0;
+          if (!q0r) cpar ^= cbit(t0, g0);
+          if (kind == PQS_OP_RZZ && !q1r) cpar ^= cbit(t1, g1);
+          if (q0r && q1r) {
+            if (fb) {
+              SW4(r0, SW4C(r1, reg_rzz<B_, C_, true>(a, t)));
+            } else {
+              SW4(r0, SW4C(r1, reg_rzz<B_, C_, false>(a, t)));
+            }
+          } else if (q0r || q1r) {
+            const int rr = q0r ? r0 : r1;
+            // odd constant parity swaps the two phases: diag(e^{+}, e^{-})
+            const R tt = cpar ? -t : t;
+            if (!fb) {
+              SW4(rr, reg_rz<B_, false>(a, tt));
+            } else {
+              // form B: x (u -/+ i); swapping the sign of i is conj of the phase
+              if (cpar) {
+                SW4(rr, reg_rz<B_, true>(a, -t));
+                fscale = -fscale;
+              } else {
+                SW4(rr, reg_rz<B_, true>(a, t));
+              }
+            }
+          } else {
+            if (fb) {
+              if (cpar) reg_phase_all<true, false>(a, t);
+              else reg_phase_all<true, true>(a, t);
+            } else {
+              if (cpar) reg_phase_all<false, false>(a, t);
+              else reg_phase_all<false, true>(a, t);
+            }
+          }
+          break;
+        }
+        case PQS_OP_MAT1: {
+          const DevOp *d = ops + o;
+          const V m00{R(d->m[0]), R(d->m[1])}, m01{R(d->m[2]), R(d->m[3])};
+          const V m10{R(d->m[4]), R(d->m[5])}, m11{R(d->m[6]), R(d->m[7])};
+          if (r0 != 0xff) {
+            SW4(r0, reg_mat1<B_>(a, m00, m01, m10, m11));
+          } else {
+            reg_scale(a, cbit(t0, g0) ? m11 : m00);  // diagonal by planning
+          }
+          break;
+        }
+        case PQS_OP_CX:
+          if (r0 != 0xff) {
+            SW4(r1, SW4C(r0, reg_cx<B_, C_>(a)));
+          } else if (cbit(t0, g0)) {
+            SW4(r1, reg_x<B_>(a));
+          }
+          break;
+        case PQS_OP_CZ:
+          if (r0 != 0xff && r1 != 0xff) {
+            SW4(r0, SW4C(r1, reg_cz<B_, C_>(a)));
+          } else if (r0 != 0xff) {
+            if (cbit(t1, g1)) SW4(r0, reg_z<B_>(a));
+          } else if (r1 != 0xff) {
+            if (cbit(t0, g0)) SW4(r1, reg_z<B_>(a));
+          } else if (cbit(t0, g0) && cbit(t1, g1)) {
+            fscale = -fscale;
+          }
+          break;
+        default:
+          break;
+      }
+    }
+    if (fscale != 1.0) {
+      const R fs = R(fscale);
+#pragma unroll
+      for (int v = 0; v < NA; ++v) a[v] = V{a[v].x * fs, a[v].y * fs};
+    }
+#pragma unroll
+    for (int v = 0; v < NA; ++v) tile[laddr(v)] = a[v];
+    __syncthreads();
+  }
+
+  // ---- scatter the tile
+#pragma unroll
+  for (int m = 0; m < NA; ++m) dst[gaddr(m)] = tile[swz(tid + m * nthr)];
 }
 
 // --------------------------------------------------------------- reductions
@@ -800,7 +1273,66 @@ int plan_upload(pqs_ctx *ctx, pqs_plan *p) {
   if (!p->dev_terms.empty())
     HIPCHECK(hipMemcpy(p->d_terms, p->dev_terms.data(), p->dev_terms.size() * sizeof(DevTerm),
                        hipMemcpyHostToDevice));
+  const size_t pb = std::max<size_t>(1, p->dev_phases.size()) * sizeof(pqs::DevPhase);
+  HIPCHECK(hipMalloc(&p->d_phases, pb));
+  if (!p->dev_phases.empty())
+    HIPCHECK(hipMemcpy(p->d_phases, p->dev_phases.data(), p->dev_phases.size() * sizeof(pqs::DevPhase),
+                       hipMemcpyHostToDevice));
   p->dev_device = ctx->device;
+  return PQS_OK;
+}
+
+
+// Compiled code objects, cached per source text in memory and, when
+// PQS_JIT_CACHE (default $TMPDIR/pqs_jit_cache) is writable, on disk.
+int jit_code_cached(const std::string &src, std::vector<char> &code, std::string &log) {
+  static std::mutex mu;
+  static std::unordered_map<std::string, std::vector<char>> mem;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = mem.find(src);
+    if (it != mem.end()) {
+      code = it->second;
+      return PQS_OK;
+    }
+  }
+  uint64_t h = 1469598103934665603ull;  // FNV-1a of the source
+  for (unsigned char c : src) h = (h ^ c) * 1099511628211ull;
+  const char *env = std::getenv("PQS_JIT_CACHE");
+  const char *tmp = std::getenv("TMPDIR");
+  std::string dir = env ? env : (std::string(tmp ? tmp : "/tmp") + "/pqs_jit_cache");
+  char fname[64];
+  std::snprintf(fname, sizeof fname, "/%016llx_%zu.co", (unsigned long long)h, src.size());
+  const std::string path = dir + fname;
+  if (FILE *f = std::fopen(path.c_str(), "rb")) {
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (sz > 0) {
+      code.resize(size_t(sz));
+      if (std::fread(code.data(), 1, code.size(), f) != code.size()) code.clear();
+    }
+    std::fclose(f);
+    if (!code.empty()) {
+      std::lock_guard<std::mutex> lk(mu);
+      mem[src] = code;
+      return PQS_OK;
+    }
+  }
+  int rc = pqs::jit_compile(src, code, log);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    mem[src] = code;
+  }
+  ::mkdir(dir.c_str(), 0755);
+  const std::string tmpf = path + ".tmp" + std::to_string((unsigned long long)(uintptr_t)&code);
+  if (FILE *f = std::fopen(tmpf.c_str(), "wb")) {
+    const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
+    std::fclose(f);
+    if (ok) std::rename(tmpf.c_str(), path.c_str());
+    else std::remove(tmpf.c_str());
+  }
   return PQS_OK;
 }
 
@@ -826,8 +1358,53 @@ int launch_sweep_t(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int
   return PQS_OK;
 }
 
+template <typename R>
+int launch_sweep_reg(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64_t n_active,
+                     const void *init_tab, hipStream_t st) {
+  constexpr int MAXT = sizeof(R) == 8 ? 512 : 1024;
+  static bool attr_set[64] = {false};
+  const DevSweep &sw = p->dev_sweeps[s];
+  const int S = 1 << sw.k;
+  const int nthr = S >> pqs::kRegBits;
+  if (nthr > MAXT || nthr < 64) return fail(PQS_ERR_UNSUPPORTED, "tile size outside the register executor's range");
+  size_t lds = size_t(S) * sizeof(typename CT<R>::V) + size_t(sw.op_end - sw.op_begin) * sizeof(ROp);
+  if (lds > size_t(ctx->max_lds)) return fail(PQS_ERR_UNSUPPORTED, "tile and coefficients do not fit in LDS");
+  auto kern = sw.n_coef ? k_sweep_reg<R, MAXT, true> : k_sweep_reg<R, MAXT, false>;
+  if (lds > 65536 && ctx->device < 64 && !attr_set[ctx->device]) {
+    HIPCHECK(hipFuncSetAttribute((const void *)k_sweep_reg<R, MAXT, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+    HIPCHECK(hipFuncSetAttribute((const void *)k_sweep_reg<R, MAXT, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+    attr_set[ctx->device] = true;
+  }
+  int64_t grid = sw.n_tiles * n_active;
+  if (grid <= 0) return PQS_OK;
+  if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "grid too large: split the batch");
+  hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(nthr), lds, st, sw, (const DevOp *)p->d_ops,
+                     (const DevTerm *)p->d_terms, (const pqs::DevPhase *)p->d_phases, d_insts, n_active,
+                     (const typename CT<R>::V *)init_tab, p->n, s, int(p->dev_sweeps.size()));
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
 int launch_sweep(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64_t n_active,
                  const void *init_tab, hipStream_t st) {
+  if (p->jit_module && p->dev_device == ctx->device && s < int(p->jit_fn.size()) && p->jit_fn[s]) {
+    const DevSweep &sw = p->dev_sweeps[s];
+    const int nthr = (1 << sw.k) >> pqs::kRegBits;
+    int64_t grid = sw.n_tiles * n_active;
+    if (grid <= 0) return PQS_OK;
+    if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "grid too large: split the batch");
+    long long na = (long long)n_active;
+    void *args[] = {(void *)&d_insts, (void *)&na, (void *)&init_tab};
+    HIPCHECK(hipModuleLaunchKernel((hipFunction_t)p->jit_fn[s], unsigned(grid), 1, 1, unsigned(nthr), 1, 1, 0, st,
+                                   args, nullptr));
+    return PQS_OK;
+  }
+  if (p->reg_exec) {
+    if (p->dtype == PQS_C128) return launch_sweep_reg<double>(ctx, p, s, d_insts, n_active, init_tab, st);
+    return launch_sweep_reg<float>(ctx, p, s, d_insts, n_active, init_tab, st);
+  }
   if (p->dtype == PQS_C128) {
     switch (p->block) {
       case 256: return launch_sweep_t<double, 256>(ctx, p, s, d_insts, n_active, init_tab, st);
@@ -904,6 +1481,8 @@ int pqs_plan_destroy(pqs_plan *p) {
     (void)hipDeviceSynchronize();
     if (p->d_ops) (void)hipFree(p->d_ops);
     if (p->d_terms) (void)hipFree(p->d_terms);
+    if (p->d_phases) (void)hipFree(p->d_phases);
+    for (void *m : p->jit_modules) (void)hipModuleUnload((hipModule_t)m);
   }
   delete p;
   return PQS_OK;
@@ -1316,4 +1895,101 @@ int pqs_magic(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_
   return PQS_OK;
 }
 
+int pqs_plan_jit_source(const pqs_plan *p, char *buf, int64_t cap, int64_t *len, int32_t *n_jit_sweeps) {
+  if (!p) return fail(PQS_ERR_ARG, "plan is NULL");
+  std::vector<int> sw;
+  std::string src = pqs::jit_source(*p, sw);
+  if (len) *len = int64_t(src.size());
+  if (n_jit_sweeps) *n_jit_sweeps = int32_t(sw.size());
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(size_t(cap - 1), src.size());
+    std::memcpy(buf, src.data(), n);
+    buf[n] = 0;
+  }
+  return PQS_OK;
+}
+
+int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *p, int32_t load) {
+  if (!p) return fail(PQS_ERR_ARG, "plan is NULL");
+  if (load && !ctx) return fail(PQS_ERR_ARG, "loading needs a context");
+  if (p->jit_module) return PQS_OK;
+  std::vector<int> sw;
+  for (int s = 0; s < int(p->dev_sweeps.size()); ++s)
+    if (pqs::jit_sweep_ok(*p, s)) sw.push_back(s);
+  if (sw.empty()) return PQS_OK;
+  std::vector<std::vector<char>> code(sw.size());
+  std::vector<std::string> errs(sw.size());
+  std::vector<int> rcs(sw.size(), PQS_OK);
+  // compile the sweeps in parallel (independent hiprtc programs)
+  std::atomic<size_t> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      const size_t i = next.fetch_add(1);
+      if (i >= sw.size()) return;
+      rcs[i] = jit_code_cached(pqs::jit_source_sweep(*p, sw[i]), code[i], errs[i]);
+    }
+  };
+  unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 16u));
+  nt = std::min<unsigned>(nt, unsigned(sw.size()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto &t : pool) t.join();
+  for (size_t i = 0; i < sw.size(); ++i)
+    if (rcs[i]) return fail(PQS_ERR_HIP, "hiprtc compile failed: " + errs[i].substr(0, 2000));
+  if (!load) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  int rc = plan_upload(ctx, p);
+  if (rc) return rc;
+  std::vector<void *> mods, fns(p->dev_sweeps.size(), nullptr);
+  for (size_t i = 0; i < sw.size(); ++i) {
+    hipModule_t mod;
+    hipError_t e = hipModuleLoadData(&mod, code[i].data());
+    if (e == hipSuccess) {
+      hipFunction_t f;
+      char name[64];
+      std::snprintf(name, sizeof name, "pqs_jit_sweep_%d", sw[i]);
+      e = hipModuleGetFunction(&f, mod, name);
+      mods.push_back((void *)mod);
+      if (e == hipSuccess) fns[sw[i]] = (void *)f;
+    }
+    if (e != hipSuccess) {
+      for (void *m : mods) (void)hipModuleUnload((hipModule_t)m);
+      return hip_fail(e, "loading a specialised sweep");
+    }
+  }
+  p->jit_modules = mods;
+  p->jit_fn = fns;
+  p->jit_module = mods.empty() ? nullptr : mods[0];
+  return PQS_OK;
+}
+
+int pqs_plan_jit_sweep_source(const pqs_plan *p, int32_t s, char *buf, int64_t cap, int64_t *len) {
+  if (!p) return fail(PQS_ERR_ARG, "plan is NULL");
+  if (s < 0 || s >= int32_t(p->dev_sweeps.size())) return fail(PQS_ERR_ARG, "sweep index out of range");
+  if (!pqs::jit_sweep_ok(*p, s)) {
+    if (len) *len = 0;
+    return PQS_OK;
+  }
+  std::string src = pqs::jit_source_sweep(*p, s);
+  if (len) *len = int64_t(src.size());
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(size_t(cap - 1), src.size());
+    std::memcpy(buf, src.data(), n);
+    buf[n] = 0;
+  }
+  return PQS_OK;
+}
+
+int pqs_jit_prepare(const char *src, int64_t len) {
+  if (!src || len <= 0) return fail(PQS_ERR_ARG, "empty source");
+  std::vector<char> code;
+  std::string log;
+  int rc = jit_code_cached(std::string(src, size_t(len)), code, log);
+  if (rc) return fail(PQS_ERR_HIP, "hiprtc compile failed: " + log.substr(0, 2000));
+  return PQS_OK;
+}
+
 }  // extern "C"
+
+

@@ -1,0 +1,386 @@
+// Run-time specialisation of sweeps (hiprtc).
+//
+// The interpreter kernel (engine.hip, k_sweep_reg) dispatches every op at run
+// time on its kind and register bits; with a register array of 16 complex
+// amplitudes per thread, the compiler has to bring every specialised variant
+// back to one register assignment at each dispatch join, which costs about
+// as many register moves as the arithmetic itself.  For a plan that will run
+// many times (batched sampling, Jacobians, benchmarks) the planner instead
+// emits one straight-line kernel per sweep: register bits, tile / global bit
+// positions, op kinds and fixed matrices are compile-time constants, so the
+// whole op sequence is SSA code with no joins and no moves, and each
+// rotation costs one FMA per output component (scaled form, below).
+//
+// Rotations are applied as f * M with f = cos(a/2) and
+// M = [[1, -t], [t, 1]] (R_y), [[1, -i t], [-i t, 1]] (R_x) or
+// diag(1 -/+ i t) (R_z, R_zz), t = tan(a/2).  The scalar f is the same for
+// every amplitude; it is accumulated per phase and applied once.  |cos| is
+// clamped below at 1e-13 (sign kept), which changes an entry of the matrix
+// by at most 1e-13 only when |cos(a/2)| < 1e-13.
+#include <hip/hiprtc.h>
+
+#include <cmath>
+#include <cstdio>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace pqs {
+
+void set_error(const std::string &s);
+
+static const char *kPrelude = R"PQS(
+typedef unsigned long long u64;
+struct InstDev {
+  void *state;
+  const void *src;
+  const double *params;
+  void *alt;
+  int tangent, init, start_sweep, pingpong;
+};
+#define NA 16
+__device__ __forceinline__ V cmul(V a, V b) { return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ V cadd(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ int swz(int j) { return j ^ (((j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15); }
+template <int B> __device__ __forceinline__ void ry(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if (v & (1 << B)) continue;
+    const V x = a[v], y = a[v | (1 << B)];
+    a[v] = V{fma(-t, y.x, x.x), fma(-t, y.y, x.y)};
+    a[v | (1 << B)] = V{fma(t, x.x, y.x), fma(t, x.y, y.y)};
+  }
+}
+template <int B> __device__ __forceinline__ void rx(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if (v & (1 << B)) continue;
+    const V x = a[v], y = a[v | (1 << B)];
+    a[v] = V{fma(t, y.y, x.x), fma(-t, y.x, x.y)};
+    a[v | (1 << B)] = V{fma(t, x.y, y.x), fma(-t, x.x, y.y)};
+  }
+}
+// x (1 - i t)
+__device__ __forceinline__ V pm(V x, R t) { return V{fma(t, x.y, x.x), fma(-t, x.x, x.y)}; }
+template <int B> __device__ __forceinline__ void rz(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = pm(a[v], (v & (1 << B)) ? -t : t);
+}
+template <int B0, int B1> __device__ __forceinline__ void rzz(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = pm(a[v], (((v >> B0) ^ (v >> B1)) & 1) ? -t : t);
+}
+__device__ __forceinline__ void pall(V (&a)[NA], R t) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = pm(a[v], t);
+}
+template <int B> __device__ __forceinline__ void mat1(V (&a)[NA], V m00, V m01, V m10, V m11) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if (v & (1 << B)) continue;
+    const V x = a[v], y = a[v | (1 << B)];
+    a[v] = cadd(cmul(m00, x), cmul(m01, y));
+    a[v | (1 << B)] = cadd(cmul(m10, x), cmul(m11, y));
+  }
+}
+__device__ __forceinline__ void scale(V (&a)[NA], V d) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = cmul(a[v], d);
+}
+__device__ __forceinline__ void rscale(V (&a)[NA], R f) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) a[v] = V{a[v].x * f, a[v].y * f};
+}
+template <int BT, int BC> __device__ __forceinline__ void cxr(V (&a)[NA]) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if ((v & (1 << BT)) || !(v & (1 << BC))) continue;
+    const V x = a[v];
+    a[v] = a[v | (1 << BT)];
+    a[v | (1 << BT)] = x;
+  }
+}
+template <int BT> __device__ __forceinline__ void cxc(V (&a)[NA], int on) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    if (v & (1 << BT)) continue;
+    const V x = a[v], y = a[v | (1 << BT)];
+    a[v] = on ? y : x;
+    a[v | (1 << BT)] = on ? x : y;
+  }
+}
+template <int B0, int B1> __device__ __forceinline__ void czr(V (&a)[NA]) {
+#pragma unroll
+  for (int v = 0; v < NA; ++v)
+    if ((v & (1 << B0)) && (v & (1 << B1))) a[v] = V{-a[v].x, -a[v].y};
+}
+template <int B> __device__ __forceinline__ void czc(V (&a)[NA], int on) {
+  const R s = on ? R(-1) : R(1);
+#pragma unroll
+  for (int v = 0; v < NA; ++v)
+    if (v & (1 << B)) a[v] = V{a[v].x * s, a[v].y * s};
+}
+__device__ __forceinline__ void negall(V (&a)[NA], int on) { rscale(a, on ? R(-1) : R(1)); }
+)PQS";
+
+static constexpr int NA_() { return 1 << kRegBits; }
+static int swz_host(int j) { return j ^ (((j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15); }
+
+static std::string dbl(double v) {
+  char b[64];
+  std::snprintf(b, sizeof b, "%.17g", v);
+  std::string s(b);
+  if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
+  return s;
+}
+
+// bit expression of a qubit that is not a register bit in a phase
+static std::string cbit_expr(const DevOp &d, int which) {
+  const int t = which ? d.t1 : d.t0;
+  const int g = which ? d.g1 : d.g0;
+  std::ostringstream o;
+  if (t >= 0) o << "((toff >> " << t << ") & 1)";
+  else o << "int((base >> " << g << ") & 1ull)";
+  return o.str();
+}
+
+bool jit_sweep_ok(const pqs_plan &p, int s) {
+  if (!p.reg_exec) return false;
+  const DevSweep &ds = p.dev_sweeps[s];
+  if (ds.n_coef) return false;  // LDS phases (dense two-qubit ops, generators)
+  for (int i = ds.op_begin; i < ds.op_end; ++i) {
+    const int k = p.dev_ops[i].kind;
+    if (k != PQS_OP_MAT1 && k != PQS_OP_RX && k != PQS_OP_RY && k != PQS_OP_RZ && k != PQS_OP_CX &&
+        k != PQS_OP_CZ && k != PQS_OP_RZZ)
+      return false;
+  }
+  return true;
+}
+
+static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o);
+
+static std::string prelude(const pqs_plan &p) {
+  std::string t = p.dtype == PQS_C128 ? "typedef double R;\ntypedef double2 V;\n" : "typedef float R;\ntypedef float2 V;\n";
+  return t + kPrelude;
+}
+
+std::string jit_source_sweep(const pqs_plan &p, int s) {
+  std::ostringstream o;
+  o << prelude(p);
+  emit_sweep(p, s, o);
+  return o.str();
+}
+
+std::string jit_source(const pqs_plan &p, std::vector<int> &sweeps_out) {
+  std::ostringstream o;
+  o << prelude(p);
+  sweeps_out.clear();
+  for (int s = 0; s < int(p.dev_sweeps.size()); ++s) {
+    if (!jit_sweep_ok(p, s)) continue;
+    sweeps_out.push_back(s);
+    emit_sweep(p, s, o);
+  }
+  return o.str();
+}
+
+static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
+  const int S_total = int(p.dev_sweeps.size());
+  {
+    const DevSweep &ds = p.dev_sweeps[s];
+    const int k = ds.k;
+    const int S = 1 << k;
+    const int nthr = S >> kRegBits;
+    const int kt = k - kRegBits;
+    // angle ops of this sweep, in op order
+    std::vector<int> ang(ds.op_end - ds.op_begin, -1);
+    std::vector<int> aops;
+    for (int i = ds.op_begin; i < ds.op_end; ++i) {
+      const int kd = p.dev_ops[i].kind;
+      if (kd == PQS_OP_RX || kd == PQS_OP_RY || kd == PQS_OP_RZ || kd == PQS_OP_RZZ) {
+        ang[i - ds.op_begin] = int(aops.size());
+        aops.push_back(i);
+      }
+    }
+    const int na = int(aops.size());
+    if (na > 0) {
+      o << "__device__ const int SL" << s << "[" << na << "] = {";
+      for (int a = 0; a < na; ++a) o << (a ? "," : "") << p.dev_ops[aops[a]].slot;
+      o << "};\n__device__ const double SC" << s << "[" << na << "] = {";
+      for (int a = 0; a < na; ++a) o << (a ? "," : "") << dbl(p.dev_ops[aops[a]].scale);
+      o << "};\n__device__ const double OF" << s << "[" << na << "] = {";
+      for (int a = 0; a < na; ++a) o << (a ? "," : "") << dbl(p.dev_ops[aops[a]].offset);
+      o << "};\n";
+    }
+    o << "extern \"C\" __global__ __launch_bounds__(" << nthr << ") void pqs_jit_sweep_" << s
+      << "(const InstDev *__restrict__ insts, long long n_active, const V *__restrict__ init_tab) {\n";
+    o << "  __shared__ V tile[" << S << "];\n";
+    o << "  __shared__ double2 tf[" << (na > 0 ? na : 1) << "];\n";
+    o << "  const int tid = threadIdx.x;\n";
+    o << "  const long long gid = blockIdx.x;\n";
+    o << "  const long long inst_idx = gid >> " << (p.n - k) << ";\n";
+    o << "  const u64 tile_id = u64(gid) & " << ((ds.n_tiles - 1)) << "ull;\n";
+    o << "  if (inst_idx >= n_active) return;\n";
+    o << "  const InstDev in = insts[inst_idx];\n";
+    // base: deposit tile_id into the complement runs
+    o << "  const u64 base = 0ull";
+    for (int r = 0; r < ds.cnruns; ++r)
+      o << " | (((tile_id >> " << ds.crun_shift[r] << ") & " << ((1ull << ds.crun_len[r]) - 1) << "ull) << "
+        << ds.crun_pos[r] << ")";
+    o << ";\n";
+    o << "  const V *src; V *dst; int init = 0;\n";
+    o << "  if (in.pingpong) {\n";
+    o << "    dst = " << (((S_total - 1 - s) & 1) == 0 ? "(V *)in.state" : "(V *)in.alt") << ";\n";
+    o << "    if (" << s << " == in.start_sweep) { src = (const V *)in.src; init = in.src ? 0 : in.init; }\n";
+    o << "    else src = " << (((S_total - s) & 1) == 0 ? "(const V *)in.state" : "(const V *)in.alt") << ";\n";
+    o << "  } else {\n";
+    o << "    dst = (V *)in.state;\n";
+    o << "    if (" << s << " == in.start_sweep) { src = in.src ? (const V *)in.src : dst; init = in.src ? 0 : in.init; }\n";
+    o << "    else src = dst;\n";
+    o << "  }\n";
+    o << "  const double *params = in.params;\n";
+    // gather addresses: low kt tile bits from tid, top kRegBits from m
+    int tpos[kMaxTileBits];
+    for (int r = 0; r < ds.nruns; ++r)
+      for (int b = 0; b < ds.run_len[r]; ++b) tpos[ds.run_shift[r] + b] = ds.run_pos[r] + b;
+    o << "  const u64 dl = base";
+    for (int i = 0; i < kt; ++i) o << " | (u64((tid >> " << i << ") & 1) << " << tpos[i] << ")";
+    o << ";\n";
+    auto hp = [&](int m) {
+      unsigned long long g = 0;
+      for (int b = 0; b < kRegBits; ++b)
+        if (m & (1 << b)) g |= 1ull << tpos[kt + b];
+      return g;
+    };
+    o << "  {\n    V tmp[NA];\n";
+    o << "    if (init == 0) {\n";
+    for (int m = 0; m < NA_(); ++m) o << "      tmp[" << m << "] = src[dl | " << hp(m) << "ull];\n";
+    o << "    } else if (init == 1) {\n";
+    for (int m = 0; m < NA_(); ++m)
+      o << "      tmp[" << m << "] = V{(dl | " << hp(m) << "ull) == 0ull ? R(1) : R(0), R(0)};\n";
+    o << "    } else {\n";
+    o << "#pragma unroll\n      for (int m = 0; m < NA; ++m) {\n";
+    o << "        u64 gi = dl;\n";
+    for (int b = 0; b < kRegBits; ++b) o << "        if (m & " << (1 << b) << ") gi |= " << (1ull << tpos[kt + b]) << "ull;\n";
+    o << "        V v{R(1), R(0)};\n";
+    o << "        for (int q = 0; q < " << p.n << "; ++q) v = cmul(v, init_tab[2 * q + int((gi >> (" << (p.n - 1)
+      << " - q)) & 1ull)]);\n";
+    o << "        tmp[m] = v;\n      }\n    }\n";
+    o << "#pragma unroll\n    for (int m = 0; m < NA; ++m) tile[swz(tid + m * " << nthr << ")] = tmp[m];\n  }\n";
+    if (na > 0) {
+      o << "  for (int i = tid; i < " << na << "; i += " << nthr << ") {\n";
+      o << "    double sn, cs;\n";
+      o << "    const int sl = SL" << s << "[i];\n";
+      o << "    sincos(0.5 * (OF" << s << "[i] + (sl >= 0 ? SC" << s << "[i] * params[sl] : 0.0)), &sn, &cs);\n";
+      o << "    if (fabs(cs) < 1e-13) cs = cs < 0.0 ? -1e-13 : 1e-13;\n";
+      o << "    tf[i] = double2{sn / cs, cs};\n  }\n";
+    }
+    o << "  __syncthreads();\n";
+    // phases
+    for (int ph = ds.phase_begin; ph < ds.phase_end; ++ph) {
+      const DevPhase &P = p.dev_phases[ph];
+      o << "  {\n";
+      o << "    const int toff = 0";
+      for (int i = 0; i < P.ntb; ++i) o << " | (((tid >> " << i << ") & 1) << " << P.tb[i] << ")";
+      o << ";\n    const int pt = swz(toff);\n";
+      int kv[NA_()];
+      for (int v = 0; v < NA_(); ++v) {
+        int a = 0;
+        for (int i = 0; i < kRegBits; ++i)
+          if (v & (1 << i)) a ^= swz_host(1 << P.rb[i]);
+        kv[v] = a;
+      }
+      o << "    V a[NA];\n";
+      for (int v = 0; v < NA_(); ++v) o << "    a[" << v << "] = tile[pt ^ " << kv[v] << "];\n";
+      o << "    double fs = 1.0;\n";
+      for (int i = P.op_begin; i < P.op_end; ++i) {
+        const DevOp &d = p.dev_ops[i];
+        const int ai = ang[i - ds.op_begin];
+        const bool r0 = d.r0 >= 0, r1 = d.r1 >= 0;
+        switch (d.kind) {
+          case PQS_OP_RY:
+          case PQS_OP_RX:
+            o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; " << (d.kind == PQS_OP_RY ? "ry<" : "rx<")
+              << d.r0 << ">(a, R(c.x)); }\n";
+            break;
+          case PQS_OP_RZ:
+            if (r0)
+              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rz<" << d.r0 << ">(a, R(c.x)); }\n";
+            else
+              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; pall(a, " << cbit_expr(d, 0)
+                << " ? R(-c.x) : R(c.x)); }\n";
+            break;
+          case PQS_OP_RZZ:
+            if (r0 && r1)
+              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rzz<" << d.r0 << "," << d.r1
+                << ">(a, R(c.x)); }\n";
+            else if (r0 || r1)
+              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rz<" << (r0 ? d.r0 : d.r1) << ">(a, "
+                << cbit_expr(d, r0 ? 1 : 0) << " ? R(-c.x) : R(c.x)); }\n";
+            else
+              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; pall(a, (" << cbit_expr(d, 0) << " ^ "
+                << cbit_expr(d, 1) << ") ? R(-c.x) : R(c.x)); }\n";
+            break;
+          case PQS_OP_MAT1: {
+            auto cv = [&](int e) { return "V{R(" + dbl(d.m[2 * e]) + "), R(" + dbl(d.m[2 * e + 1]) + ")}"; };
+            if (r0)
+              o << "    mat1<" << d.r0 << ">(a, " << cv(0) << ", " << cv(1) << ", " << cv(2) << ", " << cv(3)
+                << ");\n";
+            else
+              o << "    scale(a, " << cbit_expr(d, 0) << " ? " << cv(3) << " : " << cv(0) << ");\n";
+            break;
+          }
+          case PQS_OP_CX:
+            if (r0)
+              o << "    cxr<" << d.r1 << "," << d.r0 << ">(a);\n";
+            else
+              o << "    cxc<" << d.r1 << ">(a, " << cbit_expr(d, 0) << ");\n";
+            break;
+          case PQS_OP_CZ:
+            if (r0 && r1)
+              o << "    czr<" << d.r0 << "," << d.r1 << ">(a);\n";
+            else if (r0 || r1)
+              o << "    czc<" << (r0 ? d.r0 : d.r1) << ">(a, " << cbit_expr(d, r0 ? 1 : 0) << ");\n";
+            else
+              o << "    negall(a, " << cbit_expr(d, 0) << " & " << cbit_expr(d, 1) << ");\n";
+            break;
+          default:
+            break;
+        }
+      }
+      o << "    rscale(a, R(fs));\n";
+      for (int v = 0; v < NA_(); ++v) o << "    tile[pt ^ " << kv[v] << "] = a[" << v << "];\n";
+      o << "    __syncthreads();\n  }\n";
+    }
+    for (int m = 0; m < NA_(); ++m) o << "  dst[dl | " << hp(m) << "ull] = tile[swz(tid + " << m * nthr << ")];\n";
+    o << "}\n";
+  }
+}
+
+int jit_compile(const std::string &src, std::vector<char> &code, std::string &log) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "pqs_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+    log = "hiprtcCreateProgram failed";
+    return PQS_ERR_HIP;
+  }
+  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                        "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult r = hiprtcCompileProgram(prog, 7, opts);
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  log.assign(ls, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &log[0]);
+  if (r != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return PQS_ERR_HIP;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  code.resize(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  return PQS_OK;
+}
+
+}  // namespace pqs

@@ -194,16 +194,19 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     run.clear();
     rest.clear();
     qmask blocked_any = 0, blocked_nd = 0;
+    int n_angle = 0;
     for (int ei : remaining) {
       const ExecOp &e = p.exec[ei];
       qmask touched = e.active | e.diag;
       bool dep = (touched & blocked_nd) || (e.active & blocked_any);
       bool ok = false;
+      if (n_angle >= kMaxSweepOps) dep = true;  // the sweep's LDS op table is full
       if (!dep) {
         qmask need = e.active & ~T;
         if (popc(T | need) <= k) {
           T |= need;
           ok = true;
+          ++n_angle;
         }
       }
       if (ok) {
@@ -236,11 +239,15 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
       if (so.tangent >= 0 && p.tangent_sweep[so.tangent] < 0) p.tangent_sweep[so.tangent] = int(s);
     }
 
-  // ---- device image
+  // ---- register phases and device image
+  // Tiles of >= 11 qubits use the register-phase executor: 2^kRegBits
+  // amplitudes per thread, so at least one full wave per tile.
+  p.reg_exec = p.tile_bits >= kRegBits + 6;  // at least one full wave per tile
   p.dev_ops.clear();
   p.dev_terms.clear();
   p.dev_sweeps.clear();
-  for (const Sweep &sw : p.sweeps) {
+  p.dev_phases.clear();
+  for (Sweep &sw : p.sweeps) {
     DevSweep ds{};
     int tile_index[kMaxQubits];
     for (int q = 0; q < n; ++q) tile_index[q] = -1;
@@ -268,58 +275,152 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     ds.nruns = runs(tpos, ds.run_pos, ds.run_len, ds.run_shift);
     ds.cnruns = runs(cpos, ds.crun_pos, ds.crun_len, ds.crun_shift);
     ds.n_tiles = int64_t(1) << cpos.size();
-    ds.op_begin = int(p.dev_ops.size());
-    for (int ei : sw.ops) {
-      const ExecOp &e = p.exec[ei];
-      const pqs_op &so = ops[e.first];
-      DevOp d{};
-      d.kind = e.kind == kPauliSum ? kPauliSum : so.kind;
-      d.tangent = so.tangent;
-      d.slot = so.slot;
-      d.slot2 = so.slot2;
-      d.pauli = so.pauli;
-      d.scale = so.scale;
-      d.offset = so.offset;
-      std::memcpy(d.m, so.m, sizeof d.m);
-      d.g0 = n - 1 - so.q0;
-      d.t0 = tile_index[so.q0];
-      d.g1 = so.q1 >= 0 ? n - 1 - so.q1 : -1;
-      d.t1 = so.q1 >= 0 ? tile_index[so.q1] : -1;
-      if (e.kind == kPauliSum) {
-        d.term0 = int(p.dev_terms.size());
-        d.nterms = e.n_src;
-        for (int t = e.first; t < e.first + e.n_src; ++t) {
-          const pqs_op &g = ops[t];
-          DevTerm dt{};
-          int nq = g.q1 >= 0 ? 2 : 1;
-          int qs[2] = {g.q0, g.q1};
-          int ny = 0;
-          for (int a = 0; a < nq; ++a) {
-            int q = qs[a];
-            bool x = g.pauli == 1 || g.pauli == 2;
-            bool z = g.pauli == 2 || g.pauli == 3;
-            if (g.pauli == 2) ++ny;
-            if (x) dt.xm |= 1u << tile_index[q];  // active qubits are in the tile
-            if (z) {
-              if (tile_index[q] >= 0) dt.zm_tile |= 1u << tile_index[q];
-              else dt.zm_glob |= uint64_t(1) << (n - 1 - q);
+
+    // Phases: LDS ops (two-qubit dense, generators) run alone and order
+    // everything around them; between them, greedy register phases of at
+    // most kRegBits qubits with the same dependency rule as the sweeps.
+    struct PhaseSpec {
+      bool lds;
+      std::vector<int> ops;
+      qmask regs;
+    };
+    std::vector<PhaseSpec> phases;
+    if (p.reg_exec) {
+      auto reg_capable = [](int kind) {
+        return kind == PQS_OP_MAT1 || kind == PQS_OP_RX || kind == PQS_OP_RY || kind == PQS_OP_RZ ||
+               kind == PQS_OP_CX || kind == PQS_OP_CZ || kind == PQS_OP_RZZ;
+      };
+      size_t i = 0;
+      while (i < sw.ops.size()) {
+        if (!reg_capable(p.exec[sw.ops[i]].kind)) {
+          phases.push_back({true, {sw.ops[i]}, 0});
+          ++i;
+          continue;
+        }
+        size_t j = i;
+        while (j < sw.ops.size() && reg_capable(p.exec[sw.ops[j]].kind)) ++j;
+        std::vector<int> rem(sw.ops.begin() + i, sw.ops.begin() + j);
+        while (!rem.empty()) {
+          qmask R = 0, blocked_any = 0, blocked_nd = 0;
+          std::vector<int> run, rest;
+          for (int ei : rem) {
+            const ExecOp &e = p.exec[ei];
+            qmask touched = e.active | e.diag;
+            bool dep = (touched & blocked_nd) || (e.active & blocked_any);
+            if (!dep && popc(R | e.active) <= kRegBits) {
+              R |= e.active;
+              run.push_back(ei);
+            } else {
+              rest.push_back(ei);
+              blocked_any |= touched;
+              blocked_nd |= e.active;
             }
           }
-          // coef * i^ny
-          double cr = g.m[0], ci = g.m[1];
-          for (int r = 0; r < ny; ++r) {
-            double t2 = cr;
-            cr = -ci;
-            ci = t2;
-          }
-          dt.cre = cr;
-          dt.cim = ci;
-          p.dev_terms.push_back(dt);
+          if (run.empty()) return fail(PQS_ERR_UNSUPPORTED, "phase planner made no progress");
+          phases.push_back({false, run, R});
+          rem.swap(rest);
         }
+        i = j;
       }
-      p.dev_ops.push_back(d);
+      if (int(p.dev_phases.size() + phases.size()) > (1 << 24)) return fail(PQS_ERR_UNSUPPORTED, "too many phases");
+    } else {
+      phases.push_back({false, sw.ops, 0});
     }
+
+    ds.op_begin = int(p.dev_ops.size());
+    ds.phase_begin = int(p.dev_phases.size());
+    int n_coef = 0;
+    std::vector<int> exec_order;
+    for (const PhaseSpec &ph : phases) {
+      DevPhase dp{};
+      dp.op_begin = int(p.dev_ops.size());
+      dp.lds = ph.lds ? 1 : 0;
+      int reg_index[kMaxTileBits];
+      for (int t = 0; t < kMaxTileBits; ++t) reg_index[t] = -1;
+      if (p.reg_exec && !ph.lds) {
+        // register bits: the phase's qubits, then the lowest other tile bits
+        int nr = 0;
+        for (int q = 0; q < n; ++q)
+          if ((ph.regs >> q) & 1) {
+            reg_index[tile_index[q]] = nr;
+            dp.rb[nr++] = tile_index[q];
+          }
+        for (int t = 0; t < ds.k && nr < kRegBits; ++t)
+          if (reg_index[t] < 0) {
+            reg_index[t] = nr;
+            dp.rb[nr++] = t;
+          }
+        dp.ntb = 0;
+        for (int t = 0; t < ds.k; ++t)
+          if (reg_index[t] < 0) dp.tb[dp.ntb++] = t;
+      }
+      for (int ei : ph.ops) {
+        exec_order.push_back(ei);
+        const ExecOp &e = p.exec[ei];
+        const pqs_op &so = ops[e.first];
+        DevOp d{};
+        d.kind = e.kind == kPauliSum ? kPauliSum : so.kind;
+        d.tangent = so.tangent;
+        d.slot = so.slot;
+        d.slot2 = so.slot2;
+        d.pauli = so.pauli;
+        d.scale = so.scale;
+        d.offset = so.offset;
+        std::memcpy(d.m, so.m, sizeof d.m);
+        d.g0 = n - 1 - so.q0;
+        d.t0 = tile_index[so.q0];
+        d.g1 = so.q1 >= 0 ? n - 1 - so.q1 : -1;
+        d.t1 = so.q1 >= 0 ? tile_index[so.q1] : -1;
+        d.r0 = d.t0 >= 0 ? reg_index[d.t0] : -1;
+        d.r1 = d.t1 >= 0 ? reg_index[d.t1] : -1;
+        d.coef = -1;
+        if (p.reg_exec && !ph.lds &&
+            (d.kind == PQS_OP_RX || d.kind == PQS_OP_RY || d.kind == PQS_OP_RZ || d.kind == PQS_OP_RZZ))
+          d.coef = n_coef++;
+        if (e.kind == kPauliSum) {
+          d.term0 = int(p.dev_terms.size());
+          d.nterms = e.n_src;
+          for (int t = e.first; t < e.first + e.n_src; ++t) {
+            const pqs_op &g = ops[t];
+            DevTerm dt{};
+            int nq = g.q1 >= 0 ? 2 : 1;
+            int qs[2] = {g.q0, g.q1};
+            int ny = 0;
+            for (int a = 0; a < nq; ++a) {
+              int q = qs[a];
+              bool x = g.pauli == 1 || g.pauli == 2;
+              bool z = g.pauli == 2 || g.pauli == 3;
+              if (g.pauli == 2) ++ny;
+              if (x) dt.xm |= 1u << tile_index[q];  // active qubits are in the tile
+              if (z) {
+                if (tile_index[q] >= 0) dt.zm_tile |= 1u << tile_index[q];
+                else dt.zm_glob |= uint64_t(1) << (n - 1 - q);
+              }
+            }
+            double cr = g.m[0], ci = g.m[1];  // coef * i^ny
+            for (int r = 0; r < ny; ++r) {
+              double t2 = cr;
+              cr = -ci;
+              ci = t2;
+            }
+            dt.cre = cr;
+            dt.cim = ci;
+            p.dev_terms.push_back(dt);
+          }
+        }
+        p.dev_ops.push_back(d);
+      }
+      dp.op_end = int(p.dev_ops.size());
+      if (p.reg_exec) p.dev_phases.push_back(dp);
+    }
+    sw.ops = exec_order;  // report the execution order
     ds.op_end = int(p.dev_ops.size());
+    ds.phase_end = int(p.dev_phases.size());
+    if (!p.reg_exec) ds.phase_begin = ds.phase_end = 0;
+    ds.n_coef = 0;
+    for (const PhaseSpec &ph : phases)
+      if (ph.lds) ds.n_coef = 1;
+    (void)n_coef;
     p.dev_sweeps.push_back(ds);
   }
   return PQS_OK;

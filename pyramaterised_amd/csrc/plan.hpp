@@ -38,7 +38,8 @@ struct alignas(16) DevOp {
   int32_t t0, t1;     // tile-local bit index of q0 / q1, -1 if outside the tile
   int32_t g0, g1;     // global bit position of q0 / q1, -1 if none
   int32_t term0, nterms;
-  int32_t pauli, pad;
+  int32_t pauli, coef;  // coef: index of (cos, sin) in the sweep's table, -1 none
+  int32_t r0, r1;       // register index of q0 / q1 in the op's phase, -1 if not
   double scale, offset;
   double m[32];
 };
@@ -51,6 +52,21 @@ struct alignas(16) DevTerm {
   double cre, cim;  // coefficient including i^{#Y}
 };
 
+// Register-phase executor: each thread holds 2^kRegBits amplitudes of the
+// tile, over kRegBits tile bits chosen per phase.
+constexpr int kRegBits = 4;
+constexpr int kMaxSweepOps = 896;  // op records staged in LDS per sweep (28 KiB)
+
+// One phase of a sweep: either register resident (ops on rb[] bits run in
+// VGPRs) or an LDS phase (one op that needs the whole tile).
+struct DevPhase {
+  int32_t op_begin, op_end;
+  int32_t lds;               // 1: LDS phase
+  int32_t rb[kRegBits];      // tile-local bit of register bit i
+  int32_t ntb;               // number of thread bits
+  int32_t tb[kMaxTileBits];  // tile-local bit of thread bit i
+};
+
 struct DevSweep {
   int32_t op_begin, op_end;
   int32_t k;          // tile bits
@@ -59,6 +75,8 @@ struct DevSweep {
   int32_t cnruns;     // runs of the complement (tile id) positions
   int32_t crun_pos[kMaxQubits], crun_len[kMaxQubits], crun_shift[kMaxQubits];
   int64_t n_tiles;
+  int32_t phase_begin, phase_end;  // register phases (reg executor only)
+  int32_t n_coef;                  // 1: the sweep has LDS phases (dense 2q ops, generators)
 };
 
 std::string plan_error(); // last planner error (thread local)
@@ -81,14 +99,27 @@ struct pqs_plan {
   std::vector<pqs::DevOp> dev_ops;
   std::vector<pqs::DevTerm> dev_terms;
   std::vector<pqs::DevSweep> dev_sweeps;
+  std::vector<pqs::DevPhase> dev_phases;
+  bool reg_exec = false;  // sweeps carry register phases
 
   // device copy (per plan, on the device of the first context that ran it)
   int dev_device = -1;
   void *d_ops = nullptr;
   void *d_terms = nullptr;
+  void *d_phases = nullptr;
+
+  // run-time specialised sweep kernels (hiprtc), per plan on one device
+  void *jit_module = nullptr;        // non-null once loaded
+  std::vector<void *> jit_modules;   // one module per specialised sweep
+  std::vector<void *> jit_fn;        // per sweep; null: interpreted
 };
 
 namespace pqs {
 // Build sweeps + the device image; returns PQS_OK or an error status.
 int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_opts *opts);
+// Specialised kernel source for the sweeps that qualify (listed in sweeps).
+std::string jit_source(const pqs_plan &p, std::vector<int> &sweeps);
+std::string jit_source_sweep(const pqs_plan &p, int s);
+bool jit_sweep_ok(const pqs_plan &p, int s);
+int jit_compile(const std::string &src, std::vector<char> &code, std::string &log);
 }

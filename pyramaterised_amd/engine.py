@@ -5,6 +5,10 @@ the hot path runs in libpqs's HIP kernels.  Every function here requires a
 gfx950 GPU and raises ``EngineError`` otherwise.
 """
 import ctypes as C
+import os
+import subprocess
+import sys
+import tempfile
 import threading
 
 import numpy as np
@@ -71,6 +75,34 @@ def context():
     return c
 
 
+_CHILD = r"""
+import ctypes, sys
+lib = ctypes.CDLL(sys.argv[1])
+lib.pqs_jit_prepare.argtypes = [ctypes.c_char_p, ctypes.c_int64]
+for path in sys.argv[2:]:
+    src = open(path, "rb").read()
+    if lib.pqs_jit_prepare(src, len(src)) != 0:
+        sys.exit(1)
+"""
+
+
+def _compile_in_children(srcs, workers):
+    """Compile kernel sources into the code-object cache with child
+    processes that load only libpqs (never the GPU)."""
+    with tempfile.TemporaryDirectory() as d:
+        paths = []
+        for i, s in enumerate(srcs):
+            p = os.path.join(d, f"s{i}.hip")
+            with open(p, "wb") as f:
+                f.write(s)
+            paths.append(p)
+        groups = [paths[i::workers] for i in range(min(workers, len(paths)))]
+        procs = [subprocess.Popen([sys.executable, "-c", _CHILD, L.LIB_PATH] + g) for g in groups]
+        for pr in procs:
+            pr.wait()
+        # failures are reported by the in-process compile that follows
+
+
 def max_tile_bits(dtype_c):
     """Largest tile (qubits) whose amplitudes fit in one workgroup's LDS."""
     lds = context().max_lds
@@ -124,6 +156,42 @@ class Plan:
             check(L.lib().pqs_plan_sweep(self.handle, s, tq, C.byref(nt), oi, C.byref(no)))
             out.append((list(tq[:nt.value]), list(oi[:no.value])))
         return out
+
+    def jit(self, load=True, workers=None):
+        """Specialise the plan's sweeps into straight-line kernels (hiprtc);
+        with load, later runs on this device use them.  hiprtc serialises
+        compiles within a process, so the sweeps are compiled by ``workers``
+        child processes (no GPU use) into the shared code-object cache
+        first."""
+        srcs = self.jit_sweep_sources()
+        if workers is None:
+            workers = min(16, os.cpu_count() or 1)
+        if workers > 1 and len(srcs) > 1:
+            _compile_in_children(srcs, workers)
+        ctx = context() if load else None
+        check(L.lib().pqs_plan_jit(ctx.handle if ctx else None, self.handle, 1 if load else 0))
+        self.jitted = bool(load)
+        return self
+
+    def jit_sweep_sources(self):
+        out = []
+        for s in range(self.n_sweeps):
+            n = C.c_int64()
+            check(L.lib().pqs_plan_jit_sweep_source(self.handle, s, None, 0, C.byref(n)))
+            if n.value == 0:
+                continue
+            buf = C.create_string_buffer(n.value + 1)
+            check(L.lib().pqs_plan_jit_sweep_source(self.handle, s, buf, n.value + 1, C.byref(n)))
+            out.append(buf.value)
+        return out
+
+    def jit_source(self):
+        n = C.c_int64()
+        ns = C.c_int32()
+        check(L.lib().pqs_plan_jit_source(self.handle, None, 0, C.byref(n), C.byref(ns)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(L.lib().pqs_plan_jit_source(self.handle, buf, n.value + 1, C.byref(n), C.byref(ns)))
+        return buf.value.decode(), ns.value
 
     def tangent_sweep(self, t):
         s = C.c_int32()

@@ -241,6 +241,8 @@ class Ket(Qobj):
         return object.__new__(cls)
 
     def __init__(self, tensor, n_qubits=None):
+        if isinstance(tensor, Ket):
+            return  # Qobj(ket) already returned a copy from __new__
         t = tensor.reshape(-1)
         self.t = t
         self.n = n_qubits if n_qubits is not None else _qubits_of(t.numel())

@@ -1,0 +1,119 @@
+"""Run-time specialised sweep kernels (hiprtc) against the CPU oracle and
+against the interpreted kernels."""
+import numpy as np
+import pytest
+
+from oracle import circuits as OC
+from oracle import measures as OM
+from oracle import sim as OS
+from tests.helpers import infidelity, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pyqc():
+    import pyramaterised_amd
+    return pyramaterised_amd
+
+
+def _build(pyqc, n, layers, jit, **kw):
+    qc = pyqc.PQC(n, jit=jit, **kw)
+    for l in layers:
+        qc.add_layer(l)
+    return qc
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 1e-5)])
+def test_jit_brickwork_22q(pyqc, dtype, tol):
+    from pyramaterised_amd import templates as T
+    n, p = 22, 30
+    qc = _build(pyqc, n, T.ry_cz_brickwork_layers(p, n), True, dtype=dtype)
+    assert qc.plan().jitted
+    th = np.random.default_rng(4).random(qc.n_true_params) * 2 * np.pi
+    st = qc.run(list(th)).full().ravel()
+    ref = OS.run(OC.ry_cz_brickwork(p, n), th, n)
+    assert infidelity(st, ref) < tol
+
+
+def test_jit_clifford_he_16q_batch_matches_interpreter_and_oracle(pyqc):
+    from pyramaterised_amd import templates as T
+    n, p, S = 16, 6, 6
+    qj = _build(pyqc, n, T.clifford_HE_layers(p, n), True)
+    qi = _build(pyqc, n, T.clifford_HE_layers(p, n), False)
+    params = np.random.default_rng(5).random((S, qj.n_true_params)) * 2 * np.pi
+    sj = qj.run_batch(params).cpu().numpy()
+    si = qi.run_batch(params).cpu().numpy()
+    og = OC.clifford_he(p, n)
+    for s in range(S):
+        ref = OS.run(og, params[s], n)
+        assert infidelity(sj[s], ref) < 1e-12
+        assert np.max(np.abs(sj[s] - si[s])) < 1e-13
+
+
+def test_jit_all_reg_kinds_multi_sweep(pyqc):
+    """R_x, R_y, R_z, fixed 1q, CNOT, CZ, R_zz with qubits inside and outside
+    the tile and register bits, many sweeps (small tiles)."""
+    rng = np.random.default_rng(6)
+    n = 14
+    kinds1 = ["R_x", "R_y", "R_z", "H", "T", "S", "X"]
+    kinds2 = ["CNOT", "CZ", "R_zz"]
+    spec = []
+    for _ in range(6):
+        for q in range(n):
+            spec.append((kinds1[rng.integers(len(kinds1))], q))
+        for _ in range(n):
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            spec.append((kinds2[rng.integers(len(kinds2))], (a, b)))
+    pk, og = [], []
+    for name, q in spec:
+        if isinstance(q, tuple):
+            pk.append(getattr(pyqc, name)(list(q), n))
+            og.append(getattr(OC, name)(q, n))
+        else:
+            pk.append(getattr(pyqc, name)(q, n))
+            og.append(getattr(OC, name)(q, n))
+    for tile in (10, 11, 13):
+        qc = _build(pyqc, n, [pk], True, tile_bits=tile, low_bits=3)
+        plan = qc.plan()
+        assert plan.jitted and plan.n_sweeps > 1
+        th = rng.random(qc.n_true_params) * 2 * np.pi
+        st = qc.run(list(th)).full().ravel()
+        ref = OS.run(og, th, n)
+        assert infidelity(st, ref) < 1e-12
+        assert np.max(np.abs(st - ref)) < 1e-12
+
+
+def test_jit_jacobian_and_qfim(pyqc):
+    """Forward Jacobian mixing specialised sweeps with interpreted generator
+    sweeps; QFIM against the reference algorithm."""
+    from pyramaterised_amd import templates as T
+    n, p = 14, 2
+    qc = _build(pyqc, n, T.clifford_HE_layers(p, n), True, tile_bits=11, low_bits=3)
+    th = np.random.default_rng(7).random(qc.n_true_params) * 2 * np.pi
+    qc.state = qc.run(list(th))
+    psi, J = qc.jacobian()
+    og = OC.clifford_he(p, n)
+    ref = OS.run(og, th, n)
+    dref = OS.derivative_states(og, th, n)
+    Jh = J.cpu().numpy()
+    for t in range(0, len(dref), 7):
+        assert np.max(np.abs(Jh[t] - dref[t])) < 1e-12
+    F = pyqc.measure.Measurements(qc).get_QFI()
+    assert rel_fro(F, OM.qfim(dref, ref)) < 1e-9
+
+
+def test_jit_product_init_and_edge_angles(pyqc):
+    """|+>^n start (product init) and angles at multiples of pi, where
+    cos(a/2) is ~0 and the scaled rotation form is clamped."""
+    from pyramaterised_amd import templates as T
+    n, p = 12, 3
+    qc = _build(pyqc, n, T.clifford_HE_layers(p, n), True)
+    qc.set_initial_state(T.plus_state())
+    th = np.random.default_rng(8).choice([0.0, np.pi / 2, np.pi, 3 * np.pi / 2, 2 * np.pi, 3 * np.pi],
+                                         size=qc.n_true_params)
+    st = qc.run(list(th)).full().ravel()
+    init = OS.product_state(np.array([1, 1]) / np.sqrt(2), n)
+    ref = OS.run(OC.clifford_he(p, n), th, n, init)
+    assert infidelity(st, ref) < 1e-12
+    assert np.max(np.abs(st - ref)) < 1e-12
