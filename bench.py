@@ -28,7 +28,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_HBM = 8.0e12  # MI355X HBM3E bytes/s
+PEAK_HBM = 8.0e12   # MI355X HBM3E bytes/s
+PEAK_FP64 = 78.6e12  # MI355X FP64 flop/s (vector and matrix rates are equal)
 
 
 def log(*a):
@@ -46,6 +47,16 @@ def build_c2(pyqc, n=16, p=20):
     for l in T.clifford_HE_layers(p, n):
         qc.add_layer(l)
     return qc
+
+
+def algorithmic_flops(qc):
+    """Flops of one state evolution by the per-gate counts of a dense
+    statevector update: rotations (R_x, R_y, R_z, R_zz) 6 per amplitude,
+    fixed dense 1q 14, dense 2q 30, CNOT / CZ 0 (permutations and signs)."""
+    from pyramaterised_amd import _lib as L
+    per = {L.OP_RX: 6, L.OP_RY: 6, L.OP_RZ: 6, L.OP_RZZ: 6, L.OP_RXX: 6, L.OP_RYY: 6,
+           L.OP_MAT1: 14, L.OP_MAT2: 30, L.OP_FSIM: 30}
+    return sum(per.get(o.kind, 0) for o in qc.ops()) * float(2 ** qc.n_qubits)
 
 
 def sweep_bytes(plan, batch, amp_bytes, init_reads=False):
@@ -119,14 +130,15 @@ def headline(pyqc, args):
     gates = S * 940
     expr = m.expr(F, 2 ** 16, bins=75)
     bytes_run = sweep_bytes(plan, S, 16)
+    flops_run = algorithmic_flops(qc) * S
     launches = plan.n_sweeps
-    achieved = bytes_run / (evolve_ms * 1e-3)
     return {
         "ms_per_step": ms_step, "gates_per_s": gates / (ms_step * 1e-3), "evolve_ms": evolve_ms,
-        "n_sweeps": plan.n_sweeps, "sweep_launch_ms": evolve_ms / launches,
-        "bytes_per_launch": bytes_run / launches, "achieved_Bps": achieved, "expr_kl": float(expr),
-        "mean_F": float(F.mean().item()), "counts_nonzero": int((counts > 0).sum()), "S": S, "P": P,
-        "jit": bool(getattr(plan, "jitted", False)), "jit_compile_s": jit_s,
+        "n_sweeps": plan.n_sweeps, "tile_bits": plan.tile_bits, "sweep_launch_ms": evolve_ms / launches,
+        "bytes_per_launch": bytes_run / launches, "achieved_Bps": bytes_run / (evolve_ms * 1e-3),
+        "flops_per_launch": flops_run / launches, "achieved_flops": flops_run / (evolve_ms * 1e-3),
+        "expr_kl": float(expr), "mean_F": float(F.mean().item()), "counts_nonzero": int((counts > 0).sum()),
+        "S": S, "P": P, "jit": bool(getattr(plan, "jitted", False)), "plan_and_tune_s": jit_s,
     }
 
 
@@ -139,7 +151,7 @@ def leg_28q(pyqc, dtype, steps=3):
     for l in T.ry_cz_brickwork_layers(p, n):
         qc.add_layer(l)
     th = np.random.default_rng(7).random(qc.n_true_params) * 2 * np.pi
-    plan = qc.plan()  # specialised kernels (28 qubits)
+    plan = qc.plan()  # specialised kernels, tile shape tuned on first use
     amp = 16 if dtype == "complex128" else 8
     out = torch.empty((1, 2 ** n), dtype=E.CDTYPE[plan.dtype], device="cuda")
     params = torch.as_tensor(th, device="cuda")
@@ -157,6 +169,7 @@ def leg_28q(pyqc, dtype, steps=3):
     del out
     torch.cuda.empty_cache()
     return {"dtype": dtype, "ms": ms, "gates_per_s": 1245 / (ms * 1e-3), "n_sweeps": plan.n_sweeps,
+            "tile_bits": plan.tile_bits, "sweep_launch_ms": ms / plan.n_sweeps,
             "jit": bool(getattr(plan, "jitted", False)),
             "GBps": b / (ms * 1e-3) / 1e9, "hbm_frac": b / (ms * 1e-3) / PEAK_HBM, "norm": norm}
 
@@ -253,6 +266,20 @@ def speedups_16q(pyqc, args, head):
             "qfim16_plain_resim_s": t_resim, "qfim16_overlap_call_us": t_ov * 1e6, "jac_forward_vs_plain_maxdiff": jdiff}
 
 
+def traffic_per_launch(head):
+    """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE on this workload, scaled by the same counters' reading for a
+    plain copy of known size (profiles/traffic_headline.json); None when that
+    measurement is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_headline.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return float(t["bytes_per_launch"]) * head["S"] / float(t["states"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def cpu_baseline(args):
     from oracle import cpu_ref
     n, p = 16, 20
@@ -315,10 +342,15 @@ def main():
         "config": {"workload": "batched expressibility: 16-qubit HEA (R_y,R_z + CNOT CHAIN) x 20 layers, "
                                "940 gates, 640 params, 5000 pairs (10,000 states), 75-bin KL vs Haar(2^16)",
                    "n_qubits": 16, "layers": 20, "states": head["S"], "pairs": head["S"] // 2},
-        "roofline": {"bound": "hbm", "achieved": head["achieved_Bps"] / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": head["achieved_Bps"] / PEAK_HBM, "traffic": None,
-                     "kernel": "k_sweep<double>", "launch_ms": head["sweep_launch_ms"],
-                     "bytes_per_launch": head["bytes_per_launch"]},
+        # The headline's sweeps hold 10,000 resident 1 MiB states and run
+        # ~10 register phases per LDS tile: FP64 VALU + LDS bound, so the
+        # compute roofline applies (FP64 vector peak = FP64 matrix peak).
+        "roofline": {"bound": "mfma", "achieved": head["achieved_flops"] / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": head["achieved_flops"] / PEAK_FP64,
+                     "traffic": traffic_per_launch(head),
+                     "kernel": "pqs_jit_sweep_* (run-time specialised sweeps, all launches of one evolution)",
+                     "launch_ms": head["sweep_launch_ms"], "flops_per_launch": head["flops_per_launch"],
+                     "hbm_GBps": head["achieved_Bps"] / 1e9, "hbm_frac": head["achieved_Bps"] / PEAK_HBM},
         "cpu_baseline": cpu,
         "headline_detail": head,
     }

@@ -87,7 +87,17 @@ typedef struct pqs_plan_opts {
   int32_t low_bits;     /* lowest bit positions forced into every tile        */
   int32_t block;        /* threads per workgroup (256, 512 or 1024)           */
   int32_t dtype;        /* pqs_dtype                                          */
+  int32_t flags;        /* PQS_PLAN_* tuning switches (0: defaults)           */
 } pqs_plan_opts;
+
+/* Planner tuning switches (pqs_plan_opts.flags). */
+enum {
+  PQS_PLAN_LANES_ASCENDING = 1, /* thread bits in ascending order, not bank-aware */
+  PQS_PLAN_NO_SOFT_REGS = 2,    /* do not pull CNOT / CZ qubits into registers   */
+  PQS_PLAN_JIT_PERSISTENT = 4   /* specialised kernels: persistent blocks that
+                                   prefetch their next tile into registers (pays
+                                   off only at one resident block per CU)         */
+};
 
 /* One instance of a batched run. */
 typedef struct pqs_instance {

@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libpqs.so")
 
 PQS_OK = 0
 PQS_C128, PQS_C64 = 0, 1
+# pqs_plan_opts.flags
+PQS_PLAN_LANES_ASCENDING, PQS_PLAN_NO_SOFT_REGS, PQS_PLAN_JIT_PERSISTENT = 1, 2, 4
 
 (OP_MAT1, OP_RX, OP_RY, OP_RZ, OP_MAT2, OP_CX, OP_CZ, OP_RXX, OP_RYY, OP_RZZ,
  OP_FSIM, OP_GEN, OP_GEN_FSIM) = range(13)
@@ -26,7 +28,7 @@ class Op(C.Structure):
 
 class PlanOpts(C.Structure):
     _fields_ = [("tile_bits", C.c_int32), ("low_bits", C.c_int32),
-                ("block", C.c_int32), ("dtype", C.c_int32)]
+                ("block", C.c_int32), ("dtype", C.c_int32), ("flags", C.c_int32)]
 
 
 class Instance(C.Structure):

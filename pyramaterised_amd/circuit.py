@@ -24,7 +24,7 @@ from .qobj import Ket, PauliSum, ProductKet, Qobj, basis, expect, tensor
 class PQC:
     """An n-qubit, p-layer parameterised quantum circuit (circuit.py:12-198)."""
 
-    def __init__(self, n_qubits, dtype="complex128", tile_bits=0, low_bits=0, block=0, jit="auto"):
+    def __init__(self, n_qubits, dtype="complex128", tile_bits=0, low_bits=0, block=0, jit="auto", flags=0):
         self.n_qubits = n_qubits
         self.n_layers = 0
         self.layers = []
@@ -36,7 +36,7 @@ class PQC:
         self.initial_state = tensor([basis(2, 0) for _ in range(self.n_qubits)])
         self.state = self.initial_state
         self.dtype = E.dtype_code(dtype)
-        self.plan_opts = dict(tile_bits=tile_bits, low_bits=low_bits, block=block)
+        self.plan_opts = dict(tile_bits=tile_bits, low_bits=low_bits, block=block, flags=flags)
         # run-time specialised sweep kernels: "auto" uses them for batches and
         # large states, where their one-off compile pays back; PQS_JIT=0/1
         # overrides
@@ -171,12 +171,28 @@ class PQC:
         dt = self.dtype if dtype is None else E.dtype_code(dtype)
         key = (with_generators, dt)
         p = self._plans.get(key)
-        if p is None:
-            p = E.Plan(self.n_qubits, self.ops(with_generators), dtype=dt, **self.plan_opts)
-            self._plans[key] = p
-        if not getattr(p, "jitted", False) and self._want_jit(batch):
-            p.jit()
+        want = self._want_jit(batch)
+        if p is not None and (getattr(p, "jitted", False) or not want):
+            return p
+        ops = self.ops(with_generators)
+        if want and self._want_tune(with_generators):
+            # specialised kernels: time the candidate tile shapes once
+            p = E.tuned_plan(self.n_qubits, ops, dt, batch, self.init())
+        else:
+            if p is None:
+                p = E.Plan(self.n_qubits, ops, dtype=dt, **self.plan_opts)
+            if want:
+                p.jit()
+        self._plans[key] = p
         return p
+
+    def _want_tune(self, with_generators):
+        """Tile shapes are tuned unless the caller fixed them (constructor
+        options) or PQS_TUNE=0; generator plans keep the default shape."""
+        import os
+        if os.environ.get("PQS_TUNE", "1") in ("0", "", "false", "False"):
+            return False
+        return not with_generators and not any(self.plan_opts.values())
 
     def _want_jit(self, batch):
         import os

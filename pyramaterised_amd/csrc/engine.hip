@@ -1394,6 +1394,15 @@ int launch_sweep(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64
     const int nthr = (1 << sw.k) >> pqs::kRegBits;
     int64_t grid = sw.n_tiles * n_active;
     if (grid <= 0) return PQS_OK;
+    if (p->flags & PQS_PLAN_JIT_PERSISTENT) {
+      // persistent kernels: as many blocks as can be resident at once
+      int per_cu = 0;
+      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (hipFunction_t)p->jit_fn[s], nthr, 0) !=
+              hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+      grid = std::min<int64_t>(grid, int64_t(ctx->n_cu) * per_cu);
+    }
     if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "grid too large: split the batch");
     long long na = (long long)n_active;
     void *args[] = {(void *)&d_insts, (void *)&na, (void *)&init_tab};

@@ -213,38 +213,31 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       for (int a = 0; a < na; ++a) o << (a ? "," : "") << dbl(p.dev_ops[aops[a]].offset);
       o << "};\n";
     }
+    // Block b processes flat tile b (flat tile = instance * n_tiles + tile).
+    // Persistent form (PQS_PLAN_JIT_PERSISTENT): block b processes tiles b,
+    // b + G, b + 2G, ... and issues the loads of its next tile into
+    // registers before the current tile's phases, so HBM traffic overlaps
+    // compute when only one block fits per CU.
     o << "extern \"C\" __global__ __launch_bounds__(" << nthr << ") void pqs_jit_sweep_" << s
       << "(const InstDev *__restrict__ insts, long long n_active, const V *__restrict__ init_tab) {\n";
     o << "  __shared__ V tile[" << S << "];\n";
     o << "  __shared__ double2 tf[" << (na > 0 ? na : 1) << "];\n";
     o << "  const int tid = threadIdx.x;\n";
-    o << "  const long long gid = blockIdx.x;\n";
-    o << "  const long long inst_idx = gid >> " << (p.n - k) << ";\n";
-    o << "  const u64 tile_id = u64(gid) & " << ((ds.n_tiles - 1)) << "ull;\n";
-    o << "  if (inst_idx >= n_active) return;\n";
-    o << "  const InstDev in = insts[inst_idx];\n";
-    // base: deposit tile_id into the complement runs
-    o << "  const u64 base = 0ull";
+    o << "  const long long total = n_active << " << (p.n - k) << ";\n";
+    o << "  long long t = blockIdx.x;\n";
+    o << "  if (t >= total) return;\n";
+    // address / source helpers for a flat tile index
+    o << "  auto tile_base = [&](long long ft) -> u64 {\n";
+    o << "    const u64 tile_id = u64(ft) & " << (ds.n_tiles - 1) << "ull;\n";
+    o << "    return 0ull";
     for (int r = 0; r < ds.cnruns; ++r)
       o << " | (((tile_id >> " << ds.crun_shift[r] << ") & " << ((1ull << ds.crun_len[r]) - 1) << "ull) << "
         << ds.crun_pos[r] << ")";
-    o << ";\n";
-    o << "  const V *src; V *dst; int init = 0;\n";
-    o << "  if (in.pingpong) {\n";
-    o << "    dst = " << (((S_total - 1 - s) & 1) == 0 ? "(V *)in.state" : "(V *)in.alt") << ";\n";
-    o << "    if (" << s << " == in.start_sweep) { src = (const V *)in.src; init = in.src ? 0 : in.init; }\n";
-    o << "    else src = " << (((S_total - s) & 1) == 0 ? "(const V *)in.state" : "(const V *)in.alt") << ";\n";
-    o << "  } else {\n";
-    o << "    dst = (V *)in.state;\n";
-    o << "    if (" << s << " == in.start_sweep) { src = in.src ? (const V *)in.src : dst; init = in.src ? 0 : in.init; }\n";
-    o << "    else src = dst;\n";
-    o << "  }\n";
-    o << "  const double *params = in.params;\n";
-    // gather addresses: low kt tile bits from tid, top kRegBits from m
+    o << ";\n  };\n";
     int tpos[kMaxTileBits];
     for (int r = 0; r < ds.nruns; ++r)
       for (int b = 0; b < ds.run_len[r]; ++b) tpos[ds.run_shift[r] + b] = ds.run_pos[r] + b;
-    o << "  const u64 dl = base";
+    o << "  const u64 dtid = 0ull";
     for (int i = 0; i < kt; ++i) o << " | (u64((tid >> " << i << ") & 1) << " << tpos[i] << ")";
     o << ";\n";
     auto hp = [&](int m) {
@@ -253,12 +246,26 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         if (m & (1 << b)) g |= 1ull << tpos[kt + b];
       return g;
     };
-    o << "  {\n    V tmp[NA];\n";
+    o << "  auto io = [&](const InstDev &in, const V *&src, V *&dst, int &init) {\n";
+    o << "    init = 0;\n";
+    o << "    if (in.pingpong) {\n";
+    o << "      dst = " << (((S_total - 1 - s) & 1) == 0 ? "(V *)in.state" : "(V *)in.alt") << ";\n";
+    o << "      if (" << s << " == in.start_sweep) { src = (const V *)in.src; init = in.src ? 0 : in.init; }\n";
+    o << "      else src = " << (((S_total - s) & 1) == 0 ? "(const V *)in.state" : "(const V *)in.alt") << ";\n";
+    o << "    } else {\n";
+    o << "      dst = (V *)in.state;\n";
+    o << "      if (" << s << " == in.start_sweep) { src = in.src ? (const V *)in.src : dst; init = in.src ? 0 : in.init; }\n";
+    o << "      else src = dst;\n";
+    o << "    }\n  };\n";
+    o << "  auto fetch = [&](long long ft, V (&pre)[NA]) {\n";
+    o << "    const InstDev in = insts[ft >> " << (p.n - k) << "];\n";
+    o << "    const V *src; V *dst; int init;\n    io(in, src, dst, init);\n";
+    o << "    const u64 dl = tile_base(ft) | dtid;\n";
     o << "    if (init == 0) {\n";
-    for (int m = 0; m < NA_(); ++m) o << "      tmp[" << m << "] = src[dl | " << hp(m) << "ull];\n";
+    for (int m = 0; m < NA_(); ++m) o << "      pre[" << m << "] = src[dl | " << hp(m) << "ull];\n";
     o << "    } else if (init == 1) {\n";
     for (int m = 0; m < NA_(); ++m)
-      o << "      tmp[" << m << "] = V{(dl | " << hp(m) << "ull) == 0ull ? R(1) : R(0), R(0)};\n";
+      o << "      pre[" << m << "] = V{(dl | " << hp(m) << "ull) == 0ull ? R(1) : R(0), R(0)};\n";
     o << "    } else {\n";
     o << "#pragma unroll\n      for (int m = 0; m < NA; ++m) {\n";
     o << "        u64 gi = dl;\n";
@@ -266,8 +273,19 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     o << "        V v{R(1), R(0)};\n";
     o << "        for (int q = 0; q < " << p.n << "; ++q) v = cmul(v, init_tab[2 * q + int((gi >> (" << (p.n - 1)
       << " - q)) & 1ull)]);\n";
-    o << "        tmp[m] = v;\n      }\n    }\n";
-    o << "#pragma unroll\n    for (int m = 0; m < NA; ++m) tile[swz(tid + m * " << nthr << ")] = tmp[m];\n  }\n";
+    o << "        pre[m] = v;\n      }\n    }\n  };\n";
+    const bool persist = (p.flags & PQS_PLAN_JIT_PERSISTENT) != 0;
+    o << "  V pre[NA];\n  fetch(t, pre);\n";
+    if (persist) o << "  for (; t < total; t += gridDim.x) {\n";
+    else o << "  {\n";
+    o << "  const InstDev in = insts[t >> " << (p.n - k) << "];\n";
+    o << "  const V *src; V *dst; int init;\n  io(in, src, dst, init);\n";
+    o << "  const u64 base = tile_base(t);\n";
+    o << "  const u64 dl = base | dtid;\n";
+    o << "  const double *params = in.params;\n";
+    o << "  __syncthreads();\n";
+    o << "#pragma unroll\n  for (int m = 0; m < NA; ++m) tile[swz(tid + m * " << nthr << ")] = pre[m];\n";
+    if (persist) o << "  if (t + gridDim.x < total) fetch(t + gridDim.x, pre);\n";
     if (na > 0) {
       o << "  for (int i = tid; i < " << na << "; i += " << nthr << ") {\n";
       o << "    double sn, cs;\n";
@@ -281,8 +299,9 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     for (int ph = ds.phase_begin; ph < ds.phase_end; ++ph) {
       const DevPhase &P = p.dev_phases[ph];
       o << "  {\n";
+      o << "    int ptid = tid;\n    asm volatile(\"\" : \"+v\"(ptid));\n";
       o << "    const int toff = 0";
-      for (int i = 0; i < P.ntb; ++i) o << " | (((tid >> " << i << ") & 1) << " << P.tb[i] << ")";
+      for (int i = 0; i < P.ntb; ++i) o << " | (((ptid >> " << i << ") & 1) << " << P.tb[i] << ")";
       o << ";\n    const int pt = swz(toff);\n";
       int kv[NA_()];
       for (int v = 0; v < NA_(); ++v) {
@@ -354,7 +373,7 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       o << "    __syncthreads();\n  }\n";
     }
     for (int m = 0; m < NA_(); ++m) o << "  dst[dl | " << hp(m) << "ull] = tile[swz(tid + " << m * nthr << ")];\n";
-    o << "}\n";
+    o << "  }\n}\n";
   }
 }
 

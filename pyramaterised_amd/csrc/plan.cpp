@@ -133,6 +133,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
   p.low_bits = low;
   p.block = block;
   p.dtype = o.dtype;
+  p.flags = o.flags;
   p.src_ops.assign(ops, ops + n_ops);
   p.exec.clear();
   p.sweeps.clear();
@@ -290,6 +291,8 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         return kind == PQS_OP_MAT1 || kind == PQS_OP_RX || kind == PQS_OP_RY || kind == PQS_OP_RZ ||
                kind == PQS_OP_CX || kind == PQS_OP_CZ || kind == PQS_OP_RZZ;
       };
+      qmask tmask = 0;  // qubits of this sweep's tile
+      for (int q : sw.tile_qubits) tmask |= bitq(q);
       size_t i = 0;
       while (i < sw.ops.size()) {
         if (!reg_capable(p.exec[sw.ops[i]].kind)) {
@@ -307,7 +310,15 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
             const ExecOp &e = p.exec[ei];
             qmask touched = e.active | e.diag;
             bool dep = (touched & blocked_nd) || (e.active & blocked_any);
-            if (!dep && popc(R | e.active) <= kRegBits) {
+            // A CNOT control or CZ qubit held in registers makes the op a
+            // register rename; from thread bits it needs per-lane selects.
+            const qmask soft = ((e.kind == PQS_OP_CX || e.kind == PQS_OP_CZ) && !(p.flags & PQS_PLAN_NO_SOFT_REGS))
+                                   ? (e.diag & tmask)
+                                   : 0;
+            if (!dep && popc(R | e.active | soft) <= kRegBits) {
+              R |= e.active | soft;
+              run.push_back(ei);
+            } else if (!dep && popc(R | e.active) <= kRegBits) {
               R |= e.active;
               run.push_back(ei);
             } else {
@@ -342,6 +353,8 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         int nr = 0;
         for (int q = 0; q < n; ++q)
           if ((ph.regs >> q) & 1) {
+            if (tile_index[q] < 0 || nr >= kRegBits)
+              return fail(PQS_ERR_UNSUPPORTED, "internal planner error: register qubit outside the tile");
             reg_index[tile_index[q]] = nr;
             dp.rb[nr++] = tile_index[q];
           }
@@ -350,9 +363,36 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
             reg_index[t] = nr;
             dp.rb[nr++] = t;
           }
+        // Thread bits; the first ones become lane bits 0, 1, 2, ... and are
+        // ordered so that their images under the LDS swizzle (tile bit t
+        // lands on bank-group bit t mod 4) differ, spreading each group of
+        // lanes over distinct banks.
         dp.ntb = 0;
+        std::vector<int> rest;
         for (int t = 0; t < ds.k; ++t)
-          if (reg_index[t] < 0) dp.tb[dp.ntb++] = t;
+          if (reg_index[t] < 0) rest.push_back(t);
+        for (int round = 0; round < 4 && !rest.empty() && !(p.flags & PQS_PLAN_LANES_ASCENDING); ++round) {
+          int used = 0;
+          for (size_t i = 0; i < rest.size();) {
+            const int cls = rest[i] & 3;
+            if (!(used & (1 << cls))) {
+              used |= 1 << cls;
+              dp.tb[dp.ntb++] = rest[i];
+              rest.erase(rest.begin() + long(i));
+            } else {
+              ++i;
+            }
+          }
+        }
+        for (int t : rest) dp.tb[dp.ntb++] = t;
+      }
+      if (p.reg_exec && !ph.lds) {
+        // internal consistency: register + thread bits permute the tile bits
+        int seen = 0;
+        for (int i2 = 0; i2 < kRegBits; ++i2) seen |= 1 << dp.rb[i2];
+        for (int i2 = 0; i2 < dp.ntb; ++i2) seen |= 1 << dp.tb[i2];
+        if (seen != (1 << ds.k) - 1 || dp.ntb != ds.k - kRegBits)
+          return fail(PQS_ERR_UNSUPPORTED, "internal planner error: phase bits do not cover the tile");
       }
       for (int ei : ph.ops) {
         exec_order.push_back(ei);
@@ -374,6 +414,12 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         d.r0 = d.t0 >= 0 ? reg_index[d.t0] : -1;
         d.r1 = d.t1 >= 0 ? reg_index[d.t1] : -1;
         d.coef = -1;
+        if (p.reg_exec && !ph.lds) {
+          const qmask act = e.active;
+          if (((act >> so.q0) & 1) && d.r0 < 0) return fail(PQS_ERR_UNSUPPORTED, "internal planner error: active qubit not in registers");
+          if (so.q1 >= 0 && ((act >> so.q1) & 1) && d.r1 < 0)
+            return fail(PQS_ERR_UNSUPPORTED, "internal planner error: active qubit not in registers");
+        }
         if (p.reg_exec && !ph.lds &&
             (d.kind == PQS_OP_RX || d.kind == PQS_OP_RY || d.kind == PQS_OP_RZ || d.kind == PQS_OP_RZZ))
           d.coef = n_coef++;

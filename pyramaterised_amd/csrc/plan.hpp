@@ -89,6 +89,7 @@ struct pqs_plan {
   int low_bits = 0;
   int block = 0;
   int dtype = 0;
+  int flags = 0;
   int n_tangents = 0;
   std::vector<pqs_op> src_ops;
   std::vector<pqs::ExecOp> exec;

@@ -5,6 +5,8 @@ the hot path runs in libpqs's HIP kernels.  Every function here requires a
 gfx950 GPU and raises ``EngineError`` otherwise.
 """
 import ctypes as C
+import hashlib
+import json
 import os
 import subprocess
 import sys
@@ -113,6 +115,98 @@ def max_tile_bits(dtype_c):
     return k
 
 
+def jit_cache_dir():
+    """Directory of the code-object cache (PQS_JIT_CACHE, default
+    $TMPDIR/pqs_jit_cache); tuning decisions are kept beside the code."""
+    env = os.environ.get("PQS_JIT_CACHE")
+    return env if env else os.path.join(os.environ.get("TMPDIR", "/tmp"), "pqs_jit_cache")
+
+
+def tile_candidates(n, dtype_c):
+    """Tile shapes (tile bits, forced low bits, flags) the tuner times.
+
+    c128: 12-bit tiles (64 KiB of LDS, two resident blocks per CU, so one
+    block's LDS phases overlap the other's HBM traffic) with 128 B or 256 B
+    contiguous runs, and 13-bit tiles (one block per CU) in persistent form.
+    c64: 13- and 12-bit tiles (64 / 32 KiB).  Fewer forced low bits leave
+    more tile room for the circuit (fewer sweeps) at some cost in access
+    contiguity; which wins depends on the circuit and the batch.
+    """
+    if dtype_c == L.PQS_C128:
+        c = [(12, 3, 0), (12, 4, 0), (13, 3, L.PQS_PLAN_JIT_PERSISTENT)]
+    else:
+        c = [(13, 4, 0), (12, 4, 0), (13, 3, 0)]
+    out = []
+    for k, low, fl in c:
+        k = min(k, n)
+        cand = (k, min(low, k), fl)
+        if cand not in out:
+            out.append(cand)
+    return out
+
+
+def _n_params(ops):
+    return 1 + max([-1] + [max(o.slot, o.slot2) for o in ops])
+
+
+def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
+    """Build the plan for each tile shape of ``tile_candidates``, specialise
+    all of them, time one evolution of each on this GPU (random angles, up to
+    2^26 amplitudes of the batch) and return the fastest.  The choice is
+    cached in memory and in the code-object cache directory, keyed by the op
+    list, the dtype and the timed batch size."""
+    cands = tile_candidates(n_qubits, dtype_c)
+    arr = (L.Op * max(1, len(ops)))(*ops)
+    B = 1 if n_qubits >= 26 else max(1, min(int(batch), (1 << 26) >> n_qubits))
+    key = hashlib.sha1(bytes(arr) + repr((n_qubits, dtype_c, B, cands)).encode()).hexdigest()[:24]
+    choice = _tune_memo.get(key)
+    path = os.path.join(jit_cache_dir(), f"tune_{key}.json")
+    if choice is None and os.path.exists(path):
+        try:
+            with open(path) as f:
+                choice = tuple(json.load(f)["shape"])
+        except (OSError, ValueError, KeyError):
+            choice = None
+    if choice is not None and tuple(choice) in cands:
+        return Plan(n_qubits, ops, dtype_c, tile_bits=choice[0], low_bits=choice[1], flags=choice[2]).jit(workers=workers)
+    plans = [Plan(n_qubits, ops, dtype_c, tile_bits=k, low_bits=lo, flags=fl) for k, lo, fl in cands]
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    srcs = [s for p in plans for s in p.jit_sweep_sources()]
+    if workers > 1 and len(srcs) > 1:
+        _compile_in_children(srcs, workers)
+    for p in plans:
+        p.jit(workers=1)
+    dev = device()
+    g = torch.Generator(device="cpu").manual_seed(1)
+    th = (torch.rand((B, max(1, _n_params(ops))), generator=g, dtype=torch.float64) * 2 * np.pi).to(dev)
+    out = torch.empty((B, 1 << n_qubits), dtype=CDTYPE[dtype_c], device=dev)
+    times = []
+    for p in plans:
+        run(p, th, init, out=out)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(2):
+            run(p, th, init, out=out)
+        e.record()
+        e.synchronize()
+        times.append(s.elapsed_time(e))
+    del out
+    best = int(np.argmin(times))
+    _tune_memo[key] = cands[best]
+    try:
+        os.makedirs(jit_cache_dir(), exist_ok=True)
+        with open(path, "w") as f:
+            json.dump({"shape": list(cands[best]), "candidates": [list(c) for c in cands],
+                       "ms_per_run": [t / 2 for t in times], "batch": B}, f)
+    except OSError:
+        pass
+    return plans[best]
+
+
+_tune_memo = {}
+
+
 def stream_ptr():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -124,14 +218,14 @@ def ptr(t):
 class Plan:
     """A compiled op list (pqs_plan).  Host-only to build; runs on the GPU."""
 
-    def __init__(self, n_qubits, ops, dtype=L.PQS_C128, tile_bits=0, low_bits=0, block=0):
+    def __init__(self, n_qubits, ops, dtype=L.PQS_C128, tile_bits=0, low_bits=0, block=0, flags=0):
         self.n = int(n_qubits)
         self.dtype = dtype_code(dtype)
         if tile_bits == 0 and torch.cuda.is_available():
             tile_bits = max_tile_bits(self.dtype)
         self.n_ops = len(ops)
         arr = (L.Op * max(1, len(ops)))(*ops)
-        opts = L.PlanOpts(tile_bits, low_bits, block, self.dtype)
+        opts = L.PlanOpts(tile_bits, low_bits, block, self.dtype, flags)
         self.handle = C.c_void_p()
         check(L.lib().pqs_plan_create(self.n, arr, len(ops), C.byref(opts), C.byref(self.handle)))
         ns, tb, ne, nt = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()

@@ -117,3 +117,33 @@ def test_jit_product_init_and_edge_angles(pyqc):
     ref = OS.run(OC.clifford_he(p, n), th, n, init)
     assert infidelity(st, ref) < 1e-12
     assert np.max(np.abs(st - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 1e-5)])
+def test_jit_tile_shapes_persistent_and_tuner(pyqc, dtype, tol):
+    """Every tile shape the tuner times (incl. the persistent form, with more
+    tiles than resident blocks so each block loops and prefetches), and the
+    tuned default, against the oracle on a batch."""
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    n, p, S = 20, 6, 6
+    og = OC.ry_cz_brickwork(p, n)
+    rng = np.random.default_rng(12)
+    P = n * p
+    params = rng.random((S, P)) * 2 * np.pi
+    refs = [OS.run(og, params[s], n) for s in range(S)]
+    cands = E.tile_candidates(n, E.dtype_code(dtype))
+    assert any(fl & E.L.PQS_PLAN_JIT_PERSISTENT for _, _, fl in cands) or dtype == "complex64"
+    for k, low, fl in cands:
+        qc = _build(pyqc, n, T.ry_cz_brickwork_layers(p, n), True, dtype=dtype, tile_bits=k, low_bits=low, flags=fl)
+        plan = qc.plan(batch=S)
+        assert plan.jitted and plan.tile_bits == k
+        st = qc.run_batch(params).cpu().numpy()
+        for s in range(S):
+            assert infidelity(st[s], refs[s]) < tol, (k, low, fl, s)
+    qc = _build(pyqc, n, T.ry_cz_brickwork_layers(p, n), True, dtype=dtype)
+    plan = qc.plan(batch=S)
+    assert plan.jitted and plan.tile_bits in [c[0] for c in cands]
+    st = qc.run_batch(params).cpu().numpy()
+    for s in range(S):
+        assert infidelity(st[s], refs[s]) < tol

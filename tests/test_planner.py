@@ -204,3 +204,20 @@ def test_pauli_sum_algebra(pyqc):
     assert np.allclose((Xs * Y).full(), (1j * Zs).full())
     dense = pyqc.PauliSum.from_dense(pyqc.Qobj(H.full()))
     assert np.allclose(dense.full(), H.full())
+
+
+def test_specialised_kernels_compile_without_gpu(pyqc, tmp_path, monkeypatch):
+    """The run-time specialised sweep source compiles with hiprtc (no device
+    needed) for a multi-sweep plan with every register op kind."""
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    monkeypatch.setenv("PQS_JIT_CACHE", str(tmp_path))
+    n = 12
+    qc = pyqc.PQC(n, jit=False)
+    for l in T.clifford_HE_layers(2, n):
+        qc.add_layer(l)
+    qc.add_layer([pyqc.R_x(0, n), pyqc.H(3, n), pyqc.CZ([1, 9], n), pyqc.R_zz([2, 11], n), pyqc.T(5, n)])
+    plan = E.Plan(n, qc.ops(), tile_bits=10, low_bits=3)
+    srcs = plan.jit_sweep_sources()
+    assert plan.n_sweeps > 1 and len(srcs) == plan.n_sweeps
+    plan.jit(load=False, workers=1)
