@@ -308,6 +308,9 @@ def main():
     ap.add_argument("--plain-overlaps", type=int, default=2000)
     ap.add_argument("--cpu-states", type=int, default=4)
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--calib", action="store_true",
+                    help="first copy 2 GiB (complex128, 2^27 elements) three times: a plain copy of known size "
+                         "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
@@ -316,6 +319,15 @@ def main():
     import torch
     import pyramaterised_amd as pyqc
     os.environ.setdefault("OMP_NUM_THREADS", "1")
+    if args.calib:
+        a = torch.ones(2 ** 27, dtype=torch.complex128, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        sync()
+        log("calibration copy: 3 x (read 2 GiB + write 2 GiB)")
+        del a, b
+        torch.cuda.empty_cache()
     log("headline: batched expressibility, 16 qubits, 5000 pairs")
     head = headline(pyqc, args)
     log(json.dumps(head))
