@@ -266,6 +266,41 @@ def speedups_16q(pyqc, args, head):
             "qfim16_plain_resim_s": t_resim, "qfim16_overlap_call_us": t_ov * 1e6, "jac_forward_vs_plain_maxdiff": jdiff}
 
 
+def leg_serving(args):
+    """Serving mix: Poisson requests (HEA, 6-20 qubits, depth 2-40, 70% state
+    + Meyer-Wallach, 30% QFIM, complex128) through the continuous-batching
+    server, then the same jobs one at a time; results must agree."""
+    from pyramaterised_amd import serve as SV
+    reqs = SV.make_workload(args.serve_requests, rate=args.serve_rate, seed=5)
+    srv = SV.Server(keep_states=False)
+    res = srv.run(reqs, realtime=True)
+    st = SV.stats(res, srv.wall)
+    srv.close()
+    # saturation: every request queued at t = 0 (closed loop), same server
+    # state (plans and specialised kernels already built)
+    sat = SV.Server(keep_states=False)
+    sat.circuits = srv.circuits
+    res_sat = sat.run(reqs, realtime=False)
+    st_sat = SV.stats(res_sat, sat.wall)
+    sat.close()
+    alone, wall = SV.run_one_at_a_time(reqs, circuits=srv.circuits)
+    dq, dF = 0.0, 0.0
+    for a, b in zip(res, alone):
+        if a.kind == SV.STATE_MW:
+            dq = max(dq, abs(a.Q - b.Q))
+        else:
+            dF = max(dF, float(np.linalg.norm(a.qfim - b.qfim) / max(np.linalg.norm(b.qfim), 1e-300)))
+    st.update({"rate_per_s": args.serve_rate, "batches": srv.n_batches,
+               "mean_batch": float(np.mean(srv.batch_sizes)), "max_batch": int(np.max(srv.batch_sizes)),
+               "specialised_classes": sum(1 for c in srv.circuits.values() for v in c._jit.values() if v is not None),
+               "saturated_requests_per_s": st_sat["requests_per_s"], "saturated_p50_ms": st_sat["p50_ms"],
+               "saturated_mean_batch": float(np.mean(sat.batch_sizes)),
+               "one_at_a_time_s": wall, "one_at_a_time_requests_per_s": len(reqs) / wall,
+               "max_abs_dQ_mix_vs_alone": dq, "max_rel_dQFIM_mix_vs_alone": dF,
+               "qfim_requests": sum(r.kind == SV.QFIM for r in reqs)})
+    return st
+
+
 def traffic_per_launch(head):
     """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE /
     WRITE_SIZE on this workload, scaled by the same counters' reading for a
@@ -307,6 +342,8 @@ def main():
     ap.add_argument("--plain-states", type=int, default=200)
     ap.add_argument("--plain-overlaps", type=int, default=2000)
     ap.add_argument("--cpu-states", type=int, default=4)
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-rate", type=float, default=25.0)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--calib", action="store_true",
                     help="first copy 2 GiB (complex128, 2^27 elements) three times: a plain copy of known size "
@@ -340,6 +377,9 @@ def main():
         log("12-qubit TFIM QFIM")
         extra["qfim_12q_tfim"] = leg_qfim12(pyqc)
         log(json.dumps(extra["qfim_12q_tfim"]))
+        log(f"serving mix: {args.serve_requests} requests at {args.serve_rate}/s")
+        extra["serving"] = leg_serving(args)
+        log(json.dumps(extra["serving"]))
         log("16-qubit speedups vs plain path")
         extra["speedup_16q"] = speedups_16q(pyqc, args, head)
         log(json.dumps(extra["speedup_16q"]))

@@ -16,6 +16,7 @@
 // two-stage with a fixed order so that every result is bit-identical whatever
 // else runs in the same batch.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <sys/stat.h>
 
@@ -683,6 +684,16 @@ __global__ __launch_bounds__(MAXT) void k_sweep_reg(const DevSweep sw, const Dev
     r.g01 = (d->g0 & 0xff) | ((d->g1 & 0xff) << 8);
     r.t = 0.0;
     r.f = 1.0;
+    if (d->kind == pqs::kGen1) {
+      // coefficient in (t, f); Pauli in regs bits 17-18; tangent split over
+      // the high halves of t01 and g01
+      r.t = d->m[0];
+      r.f = d->m[1];
+      r.regs |= (d->pauli & 3) << 17;
+      const uint32_t tg = uint32_t(d->tangent);
+      r.t01 = int32_t(uint32_t(r.t01) | ((tg & 0xffffu) << 16));
+      r.g01 = int32_t(uint32_t(r.g01) | ((tg >> 16) << 16));
+    }
     if (d->coef >= 0) {
       double sn, cs;
       sincos(0.5 * op_angle(d, params), &sn, &cs);
@@ -813,6 +824,31 @@ __global__ __launch_bounds__(MAXT) void k_sweep_reg(const DevSweep sw, const Dev
             SW4(r0, reg_mat1<B_>(a, m00, m01, m10, m11));
           } else {
             reg_scale(a, cbit(t0, g0) ? m11 : m00);  // diagonal by planning
+          }
+          break;
+        }
+        case pqs::kGen1: {
+          const int tang = int((uint32_t(t01) >> 16) | ((uint32_t(g01) >> 16) << 16));
+          if (tang != in.tangent) break;  // uniform per block: one instance per block
+          const int pauli = (regs >> 17) & 3;
+          const V c{R(op.t), R(op.f)};
+          const V z{R(0), R(0)};
+          V m00 = z, m01 = z, m10 = z, m11 = z;
+          if (pauli == 0) {
+            m00 = m11 = c;
+          } else if (pauli == 1) {
+            m01 = m10 = c;
+          } else if (pauli == 2) {  // [[0, -i c], [i c, 0]]
+            m01 = V{c.y, -c.x};
+            m10 = V{-c.y, c.x};
+          } else {
+            m00 = c;
+            m11 = cneg(c);
+          }
+          if (r0 != 0xff) {
+            SW4(r0, reg_mat1<B_>(a, m00, m01, m10, m11));
+          } else {
+            reg_scale(a, cbit(t0 & 0xff, g0 & 0xff) ? m11 : m00);  // I or Z on a constant bit
           }
           break;
         }
@@ -1071,6 +1107,18 @@ __global__ void k_qfim_combine(const double2 *__restrict__ G, const double2 *__r
   F[e] = 4 * (g.x - rhs);
 }
 
+// F from the upper triangle of the real Gram C = Re(J J^dag) (column-major,
+// C[p + q P] for p <= q) and b_p = <psi|J_p>.
+__global__ void k_qfim_combine_sym(const double *__restrict__ C, const double2 *__restrict__ bvec, int64_t P,
+                                   double *__restrict__ F) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= P * P) return;
+  const int64_t p = e / P, q = e - p * P;
+  const int64_t lo = p < q ? p : q, hi = p < q ? q : p;
+  const double2 bp = bvec[p], bq = bvec[q];
+  F[e] = 4 * (C[lo + hi * P] - (bp.x * bq.x + bp.y * bq.y));
+}
+
 struct PauliTerm {
   uint64_t x, z;
   double cre, cim;  // includes i^{#Y}
@@ -1206,6 +1254,7 @@ struct pqs_ctx {
   size_t pin_bytes = 0;
   hipEvent_t pin_event = nullptr;
   bool pin_pending = false;
+  rocblas_handle blas = nullptr;  // created on the first large QFIM
 };
 
 namespace {
@@ -1471,6 +1520,7 @@ int pqs_ctx_destroy(pqs_ctx *ctx) {
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->pin_event) (void)hipEventDestroy(ctx->pin_event);
+  if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
   delete ctx;
   return PQS_OK;
 }
@@ -1749,6 +1799,16 @@ int gram_impl(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m
   return PQS_OK;
 }
 
+// Smallest P for which pqs_qfim forms Re(J J^dag) with rocBLAS dgemm
+// (PQS_QFIM_SYRK_MIN_P; below it the split-K Gram kernel is faster).
+int64_t qfim_syrk_min_p() {
+  static const int64_t v = [] {
+    const char *e = std::getenv("PQS_QFIM_SYRK_MIN_P");
+    return e ? std::atoll(e) : int64_t(768);
+  }();
+  return v;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1770,6 +1830,35 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
   if (P <= 0 || dim <= 0) return fail(PQS_ERR_ARG, "bad sizes");
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
+  if (dtype == PQS_C128 && P >= qfim_syrk_min_p() && 2 * dim <= 0x7fffffff) {
+    // Re(J J^dag) = R R^T with R = J viewed as a real P x 2dim matrix: one
+    // real GEMM (rocBLAS dgemm on the matrix cores), half the flops of the
+    // complex Gram.  (dsyrk would halve them again but runs at a sixth of
+    // dgemm's rate for these tall-skinny shapes on gfx950.)
+    const size_t part_b = gram_part_bytes(ctx, 1, P, dim);
+    const size_t off_c = (part_b + 255) & ~size_t(255);
+    const size_t off_b = (off_c + size_t(P * P) * sizeof(double) + 255) & ~size_t(255);
+    int rc = ws_reserve(ctx, off_b + size_t(P) * sizeof(double2));
+    if (rc) return rc;
+    char *ws = (char *)ctx->ws;
+    double *C = (double *)(ws + off_c);
+    double2 *bv = (double2 *)(ws + off_b);
+    rc = gram_impl(ctx, dtype, psi, dim, 1, jac, dim, P, dim, bv, (double2 *)ws, st);
+    if (rc) return rc;
+    if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+      ctx->blas = nullptr;
+      return fail(PQS_ERR_HIP, "rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success) return fail(PQS_ERR_HIP, "rocblas_set_stream failed");
+    const double one = 1.0, zero = 0.0;
+    if (rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, rocblas_int(P),
+                      rocblas_int(P), rocblas_int(2 * dim), &one, (const double *)jac, rocblas_int(2 * dim),
+                      (const double *)jac, rocblas_int(2 * dim), &zero, C, rocblas_int(P)) != rocblas_status_success)
+      return fail(PQS_ERR_HIP, "rocblas_dgemm failed");
+    hipLaunchKernelGGL(k_qfim_combine_sym, dim3(grid_for(P * P, 256)), dim3(256), 0, st, C, bv, P, F);
+    HIPCHECK(hipGetLastError());
+    return PQS_OK;
+  }
   // workspace: [partials | G (P x P) | b (P)]
   const size_t part_b = std::max(gram_part_bytes(ctx, P, P, dim), gram_part_bytes(ctx, 1, P, dim));
   const size_t off_g = (part_b + 255) & ~size_t(255);

@@ -153,7 +153,7 @@ bool jit_sweep_ok(const pqs_plan &p, int s) {
   for (int i = ds.op_begin; i < ds.op_end; ++i) {
     const int k = p.dev_ops[i].kind;
     if (k != PQS_OP_MAT1 && k != PQS_OP_RX && k != PQS_OP_RY && k != PQS_OP_RZ && k != PQS_OP_CX &&
-        k != PQS_OP_CZ && k != PQS_OP_RZZ)
+        k != PQS_OP_CZ && k != PQS_OP_RZZ && k != kGen1)
       return false;
   }
   return true;
@@ -348,6 +348,22 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
                 << ");\n";
             else
               o << "    scale(a, " << cbit_expr(d, 0) << " ? " << cv(3) << " : " << cv(0) << ");\n";
+            break;
+          }
+          case kGen1: {
+            // c * Pauli on q0, only in the instance carrying this tangent
+            const double cr = d.m[0], ci = d.m[1];
+            double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // m00, m01, m10, m11 (re, im)
+            if (d.pauli == 0) { m[0] = cr; m[1] = ci; m[6] = cr; m[7] = ci; }
+            else if (d.pauli == 1) { m[2] = cr; m[3] = ci; m[4] = cr; m[5] = ci; }
+            else if (d.pauli == 2) { m[2] = ci; m[3] = -cr; m[4] = -ci; m[5] = cr; }
+            else { m[0] = cr; m[1] = ci; m[6] = -cr; m[7] = -ci; }
+            auto cv = [&](int e) { return "V{R(" + dbl(m[2 * e]) + "), R(" + dbl(m[2 * e + 1]) + ")}"; };
+            o << "    if (in.tangent == " << d.tangent << ") ";
+            if (r0)
+              o << "mat1<" << d.r0 << ">(a, " << cv(0) << ", " << cv(1) << ", " << cv(2) << ", " << cv(3) << ");\n";
+            else
+              o << "scale(a, " << cbit_expr(d, 0) << " ? " << cv(3) << " : " << cv(0) << ");\n";
             break;
           }
           case PQS_OP_CX:

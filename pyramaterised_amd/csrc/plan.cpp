@@ -287,7 +287,10 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     };
     std::vector<PhaseSpec> phases;
     if (p.reg_exec) {
-      auto reg_capable = [](int kind) {
+      auto reg_capable = [&](int ei) {
+        const ExecOp &e = p.exec[ei];
+        const int kind = e.kind;
+        if (kind == kPauliSum) return e.n_src == 1 && ops[e.first].q1 < 0;  // one Pauli on one qubit
         return kind == PQS_OP_MAT1 || kind == PQS_OP_RX || kind == PQS_OP_RY || kind == PQS_OP_RZ ||
                kind == PQS_OP_CX || kind == PQS_OP_CZ || kind == PQS_OP_RZZ;
       };
@@ -295,13 +298,13 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
       for (int q : sw.tile_qubits) tmask |= bitq(q);
       size_t i = 0;
       while (i < sw.ops.size()) {
-        if (!reg_capable(p.exec[sw.ops[i]].kind)) {
+        if (!reg_capable(sw.ops[i])) {
           phases.push_back({true, {sw.ops[i]}, 0});
           ++i;
           continue;
         }
         size_t j = i;
-        while (j < sw.ops.size() && reg_capable(p.exec[sw.ops[j]].kind)) ++j;
+        while (j < sw.ops.size() && reg_capable(sw.ops[j])) ++j;
         std::vector<int> rem(sw.ops.begin() + i, sw.ops.begin() + j);
         while (!rem.empty()) {
           qmask R = 0, blocked_any = 0, blocked_nd = 0;
@@ -399,7 +402,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         const ExecOp &e = p.exec[ei];
         const pqs_op &so = ops[e.first];
         DevOp d{};
-        d.kind = e.kind == kPauliSum ? kPauliSum : so.kind;
+        d.kind = e.kind == kPauliSum ? ((p.reg_exec && !ph.lds) ? kGen1 : kPauliSum) : so.kind;
         d.tangent = so.tangent;
         d.slot = so.slot;
         d.slot2 = so.slot2;
@@ -423,7 +426,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         if (p.reg_exec && !ph.lds &&
             (d.kind == PQS_OP_RX || d.kind == PQS_OP_RY || d.kind == PQS_OP_RZ || d.kind == PQS_OP_RZZ))
           d.coef = n_coef++;
-        if (e.kind == kPauliSum) {
+        if (d.kind == kPauliSum) {
           d.term0 = int(p.dev_terms.size());
           d.nterms = e.n_src;
           for (int t = e.first; t < e.first + e.n_src; ++t) {

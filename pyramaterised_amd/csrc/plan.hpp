@@ -25,6 +25,9 @@ struct ExecOp {
   qmask diag = 0;        // qubits the op acts on diagonally (block-diagonal)
 };
 constexpr int kPauliSum = 100;
+// A one-term, one-qubit generator (coef * Pauli on q0, applied only by the
+// instance whose tangent matches) placed in a register phase: device kind.
+constexpr int kGen1 = 101;
 
 struct Sweep {
   std::vector<int> tile_qubits; // ascending bit position (pos = N-1-q)

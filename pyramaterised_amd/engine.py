@@ -4,6 +4,7 @@ PyTorch provides device memory and the current HIP stream; all arithmetic of
 the hot path runs in libpqs's HIP kernels.  Every function here requires a
 gfx950 GPU and raises ``EngineError`` otherwise.
 """
+import contextlib
 import ctypes as C
 import hashlib
 import json
@@ -67,6 +68,9 @@ class Context:
 
 
 def context():
+    o = getattr(_tls, "override", None)
+    if o is not None:
+        return o
     dev = device()
     ctxs = getattr(_tls, "ctxs", None)
     if ctxs is None:
@@ -75,6 +79,20 @@ def context():
     if c is None:
         c = ctxs[dev.index] = Context(dev.index)
     return c
+
+
+@contextlib.contextmanager
+def using(ctx, stream):
+    """Issue engine calls on ``stream`` with context ``ctx``.  A context's
+    workspace serves one stream at a time, so work spread over several
+    streams needs one context per stream."""
+    prev = getattr(_tls, "override", None)
+    _tls.override = ctx
+    try:
+        with torch.cuda.stream(stream):
+            yield
+    finally:
+        _tls.override = prev
 
 
 _CHILD = r"""

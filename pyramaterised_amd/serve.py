@@ -1,0 +1,416 @@
+"""Request serving: continuous batching of mixed-width circuit jobs on one GPU.
+
+A request is a hardware-efficient ansatz (``templates.clifford_HE_layers``:
+R_y and R_z on every qubit, then a CNOT CHAIN, per layer; P = 2 n depth
+parameters) with its own parameter vector.  It asks for either
+
+  * the final state and its Meyer-Wallach entanglement
+    (``PQC.run``, circuit.py:118-125; ``Measurements.single_Q``,
+    measure.py:226-249), or
+  * the full QFIM (``Measurements.get_QFI``, measure.py:33-71).
+
+The reference evaluates one circuit at a time in one Python process and has
+no serving layer.  Here a ``Server`` keeps a queue per width class (qubit
+count, depth, request kind) and, at every turn of its loop:
+
+  1. admits requests that have arrived, oldest class first, while their
+     device memory fits the HBM budget (states stay resident until
+     ``release``);
+  2. launches one batched evolution per admitted class (all its requests'
+     parameter vectors in one call per sweep), or one forward Jacobian + QFIM
+     per QFIM request, on the next free HIP stream; each stream has its own
+     engine context (a context's workspace serves one stream);
+  3. polls the HIP events of launched batches and retires finished requests.
+
+Requests that arrive while earlier batches are on the GPU are admitted at
+the next turn as streams and memory free up, and batches of different
+classes run concurrently on different streams.  A batch's results do not
+depend on what else is in it: every instance runs the same kernels on its
+own state.
+"""
+import time
+from collections import OrderedDict, deque
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import torch
+
+from . import engine as E
+from . import templates as T
+from .circuit import PQC
+
+STATE_MW = "state_mw"
+QFIM = "qfim"
+
+
+class Request:
+    """One job: n qubits, depth layers, kind, parameters, arrival time (s)."""
+
+    __slots__ = ("rid", "n", "depth", "kind", "params", "arrival")
+
+    def __init__(self, rid, n, depth, kind, params, arrival=0.0):
+        if kind not in (STATE_MW, QFIM):
+            raise ValueError(f"unknown request kind {kind!r}")
+        params = np.asarray(params, dtype=np.float64).reshape(-1)
+        if params.size != 2 * n * depth:
+            raise ValueError(f"request {rid}: expected {2 * n * depth} parameters, got {params.size}")
+        self.rid, self.n, self.depth, self.kind = int(rid), int(n), int(depth), kind
+        self.params, self.arrival = params, float(arrival)
+
+
+class Result:
+    """Outcome of a request.  ``state`` is a device tensor view (resident
+    until ``Server.release``); ``Q`` the Meyer-Wallach value; ``qfim`` a
+    (P, P) numpy array.  Times are seconds on the server's clock."""
+
+    __slots__ = ("rid", "kind", "n", "depth", "state", "Q", "qfim", "arrival", "launched", "done")
+
+    def __init__(self, req):
+        self.rid, self.kind, self.n, self.depth = req.rid, req.kind, req.n, req.depth
+        self.state = self.Q = self.qfim = None
+        self.arrival = req.arrival
+        self.launched = self.done = None
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+def make_workload(n_requests, rate, seed=0, n_range=(6, 20), depth_range=(2, 40), qfim_frac=0.3):
+    """A Poisson stream of ``n_requests`` requests at ``rate`` per second:
+    qubits uniform over n_range, depth uniform over depth_range (inclusive),
+    a QFIM request with probability ``qfim_frac``, parameters uniform
+    [0, 2 pi), all from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests)) if rate > 0 else np.zeros(n_requests)
+    out = []
+    for i in range(n_requests):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        d = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        kind = QFIM if rng.random() < qfim_frac else STATE_MW
+        out.append(Request(i, n, d, kind, rng.random(2 * n * d) * 2 * np.pi, float(t[i])))
+    return out
+
+
+def _to_device(a, dev):
+    """Host array -> device tensor through pinned memory, asynchronous on the
+    current stream (a pageable copy would block the host until the stream's
+    earlier work is done)."""
+    return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dev, non_blocking=True)
+
+
+class _Circuit:
+    """Compiled plans of one (n, depth) class, shared by its requests.
+
+    ``plan`` / ``gplan`` (with generators) run on the interpreted sweep
+    kernels from the first request on.  ``specialise`` compiles run-time
+    specialised kernels for them in the background (child processes, no GPU);
+    ``poll`` loads finished ones on the calling (serving) thread, after which
+    ``best`` returns them."""
+
+    def __init__(self, n, depth, dtype):
+        self.qc = PQC(n, dtype=dtype, jit=False)
+        for layer in T.clifford_HE_layers(depth, n):
+            self.qc.add_layer(layer)
+        self.init = self.qc.init()
+        self._plan = None
+        self._gplan = None
+        self._jit = {}
+        self._pending = {}
+
+    def specialise(self, which, pool, workers):
+        if which in self._jit or which in self._pending:
+            return
+        p = E.Plan(self.qc.n_qubits, self.qc.ops(which == "gplan"), dtype=self.qc.dtype)
+        srcs = p.jit_sweep_sources()
+        if not srcs:
+            self._jit[which] = None
+            return
+        self._pending[which] = (p, pool.submit(E._compile_in_children, srcs, workers))
+
+    def poll(self):
+        for which, (p, fut) in list(self._pending.items()):
+            if not fut.done():
+                continue
+            del self._pending[which]
+            try:
+                fut.result()
+                p.jit(workers=1)  # code objects come from the cache
+                self._jit[which] = p
+            except Exception:  # stay on the interpreted kernels
+                self._jit[which] = None
+
+    def best(self, which):
+        j = self._jit.get(which)
+        return j if j is not None else getattr(self, which)
+
+    @property
+    def plan(self):
+        if self._plan is None:
+            self._plan = self.qc.plan()
+        return self._plan
+
+    @property
+    def gplan(self):
+        if self._gplan is None:
+            self._gplan = self.qc.plan(with_generators=True)
+        return self._gplan
+
+
+class _Slot:
+    def __init__(self, dev):
+        self.stream = torch.cuda.Stream(device=dev)  # default priority
+        self.ctx = E.Context(dev.index)
+        self.inflight = deque()
+
+
+class _Batch:
+    __slots__ = ("reqs", "results", "event", "bytes", "out", "q", "F", "slot")
+
+
+class Server:
+    """Continuous-batching request loop on the current GPU.
+
+    hbm_budget: device bytes that admitted, unreleased requests may hold
+    (default: half of the free memory).  n_streams: concurrent HIP streams.
+    max_batch: most requests of one class per launch.  keep_states: keep
+    every state-request's state resident (counted in the budget) until
+    ``release``; otherwise only Q is returned."""
+
+    def __init__(self, hbm_budget=None, n_streams=4, max_batch=512, dtype="complex128", keep_states=True,
+                 max_inflight_per_stream=2, specialise_qfim_from=14, specialise_state_from=18, compile_workers=8):
+        self.dev = E.device()
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(self.dev)
+            hbm_budget = free // 2
+        self.budget = int(hbm_budget)
+        self.in_use = 0
+        self.dtype = E.dtype_code(dtype)
+        self.amp = 16 if self.dtype == E.L.PQS_C128 else 8
+        self.max_batch = int(max_batch)
+        self.keep_states = keep_states
+        self.max_inflight = int(max_inflight_per_stream)
+        self.slots = [_Slot(self.dev) for _ in range(max(1, n_streams))]
+        self._next_slot = 0
+        self.circuits = {}
+        self.queues = OrderedDict()  # class key -> deque of requests (FIFO)
+        self.results = {}
+        self._held = {}  # rid -> bytes held by a resident state
+        self.n_batches = 0
+        self.batch_sizes = []
+        self.peak_in_use = 0
+        # classes whose requests are large enough for specialised kernels to
+        # pay back their compile: compiled in the background
+        self.spec_qfim, self.spec_state = specialise_qfim_from, specialise_state_from
+        self.compile_workers = compile_workers
+        self._pool = ThreadPoolExecutor(max_workers=2)
+        self._compiling = set()
+        self._seen = {}
+
+    # ---------------------------------------------------------------- sizing
+    def request_bytes(self, req):
+        """Device bytes a request needs while running (and, for a kept
+        state, afterwards)."""
+        dim = 1 << req.n
+        if req.kind == STATE_MW:
+            return self.amp * dim + 64 * req.n
+        P = 2 * req.n * req.depth
+        return self.amp * dim * (P + 2) + 48 * P * P
+
+    def circuit(self, n, depth):
+        key = (n, depth)
+        c = self.circuits.get(key)
+        if c is None:
+            c = self.circuits[key] = _Circuit(n, depth, self.dtype)
+        return c
+
+    # ---------------------------------------------------------------- queue
+    def submit(self, req):
+        if self.request_bytes(req) > self.budget:
+            raise ValueError(f"request {req.rid} needs {self.request_bytes(req)} bytes, over the HBM budget "
+                             f"{self.budget}")
+        self.results[req.rid] = Result(req)
+        self.queues.setdefault((req.n, req.depth, req.kind), deque()).append(req)
+
+    def pending(self):
+        return sum(len(q) for q in self.queues.values())
+
+    def inflight(self):
+        return sum(len(s.inflight) for s in self.slots)
+
+    def release(self, rid):
+        """Drop a resident state and return its bytes to the budget."""
+        r = self.results.get(rid)
+        if r is not None:
+            r.state = None
+        self.in_use -= self._held.pop(rid, 0)
+
+    # ---------------------------------------------------------------- loop
+    def _free_slot(self):
+        for k in range(len(self.slots)):
+            s = self.slots[(self._next_slot + k) % len(self.slots)]
+            if len(s.inflight) < self.max_inflight:
+                self._next_slot = (self._next_slot + k + 1) % len(self.slots)
+                return s
+        return None
+
+    def _admit_and_launch(self, now):
+        launched = 0
+        # oldest head-of-queue first
+        keys = sorted((q[0].arrival, k) for k, q in self.queues.items() if q)
+        for _, key in keys:
+            slot = self._free_slot()
+            if slot is None:
+                break
+            q = self.queues[key]
+            take, need = [], 0
+            while q and len(take) < self.max_batch:
+                b = self.request_bytes(q[0])
+                if self.in_use + need + b > self.budget:
+                    break
+                take.append(q.popleft())
+                need += b
+                if key[2] == QFIM:
+                    break  # one Jacobian per launch
+            if not take:
+                continue
+            self.in_use += need
+            self.peak_in_use = max(self.peak_in_use, self.in_use)
+            self._launch(slot, key, take, need, now)
+            launched += 1
+        return launched
+
+    def _launch(self, slot, key, reqs, need, now):
+        n, depth, kind = key
+        c = self.circuit(n, depth)
+        which = "gplan" if kind == QFIM else "plan"
+        seen = self._seen[key] = self._seen.get(key, 0) + len(reqs)
+        if seen >= 2 and n >= (self.spec_qfim if kind == QFIM else self.spec_state):
+            # a class that repeats: its compile is likely to pay back
+            c.specialise(which, self._pool, self.compile_workers)
+            self._compiling.add((n, depth))
+        b = _Batch()
+        b.reqs, b.bytes, b.slot = reqs, need, slot
+        b.out = b.q = b.F = None
+        with E.using(slot.ctx, slot.stream):
+            if kind == STATE_MW:
+                params = _to_device(np.stack([r.params for r in reqs]), self.dev)
+                b.out = E.run(c.best("plan"), params, c.init)
+                _, b.q = E.mw(b.out, n)
+            else:
+                r = reqs[0]
+                psi, J = E.jacobian(c.best("gplan"), _to_device(r.params, self.dev), c.init)
+                b.F = E.qfim(J, psi)
+                b.out = psi
+                del J
+            b.event = torch.cuda.Event()
+            b.event.record(slot.stream)
+        for r in reqs:
+            self.results[r.rid].launched = now
+        slot.inflight.append(b)
+        self.n_batches += 1
+        self.batch_sizes.append(len(reqs))
+
+    def _retire(self, now):
+        done = 0
+        for slot in self.slots:
+            while slot.inflight and slot.inflight[0].event.query():
+                b = slot.inflight.popleft()
+                kind = b.reqs[0].kind
+                if kind == STATE_MW:
+                    qs = b.q.cpu().numpy()
+                    for i, r in enumerate(b.reqs):
+                        res = self.results[r.rid]
+                        res.Q = float(qs[i])
+                        res.done = now
+                        if self.keep_states:
+                            res.state = b.out[i]
+                            held = self.amp * (1 << r.n)
+                            self._held[r.rid] = held
+                            self.in_use -= self.request_bytes(r) - held
+                        else:
+                            self.in_use -= self.request_bytes(r)
+                else:
+                    r = b.reqs[0]
+                    res = self.results[r.rid]
+                    res.qfim = b.F.cpu().numpy()
+                    res.done = now
+                    if self.keep_states:
+                        res.state = b.out
+                        held = self.amp * (1 << r.n)
+                        self._held[r.rid] = held
+                        self.in_use -= self.request_bytes(r) - held
+                    else:
+                        self.in_use -= self.request_bytes(r)
+                done += len(b.reqs)
+        return done
+
+    def _load_compiled(self):
+        for key in list(self._compiling):
+            c = self.circuits[key]
+            c.poll()
+            if not c._pending:
+                self._compiling.discard(key)
+
+    def close(self):
+        self._pool.shutdown(wait=True)
+
+    def run(self, requests, realtime=True, clock=time.perf_counter):
+        """Serve ``requests`` (sorted by arrival).  realtime: a request
+        becomes visible at its arrival time on the server clock; otherwise
+        all are visible at once.  Returns the results in request order."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        t0 = clock()
+        i, total, finished = 0, len(reqs), 0
+        while finished < total:
+            now = clock() - t0
+            while i < total and (not realtime or reqs[i].arrival <= now):
+                self.submit(reqs[i])
+                i += 1
+            finished += self._retire(now)
+            self._load_compiled()
+            self._admit_and_launch(now)
+            if self.pending() and not self.inflight():
+                # nothing running and nothing fits: released states would
+                # have to make room
+                raise RuntimeError("HBM budget exhausted by resident states; release results")
+        self.wall = clock() - t0
+        return [self.results[r.rid] for r in requests]
+
+
+def stats(results, wall):
+    lat = np.array([r.latency for r in results])
+    return {"requests": len(results), "wall_s": wall, "requests_per_s": len(results) / wall,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3)}
+
+
+def run_one_at_a_time(requests, dtype="complex128", circuits=None):
+    """The same jobs, one request per launch sequence and a device
+    synchronisation after each (the unbatched comparator).  With a server's
+    ``circuits`` it uses the same kernels the server had specialised.
+    Returns (results, wall seconds)."""
+    dev = E.device()
+    dt = E.dtype_code(dtype)
+    circuits = {} if circuits is None else circuits
+    out = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in requests:
+        c = circuits.get((r.n, r.depth))
+        if c is None:
+            c = circuits[(r.n, r.depth)] = _Circuit(r.n, r.depth, dt)
+        res = Result(r)
+        p = torch.as_tensor(r.params, device=dev)
+        if r.kind == STATE_MW:
+            st = E.run(c.best("plan"), p, c.init)
+            _, q = E.mw(st, r.n)
+            res.state, res.Q = st[0], float(q.item())
+        else:
+            psi, J = E.jacobian(c.best("gplan"), p, c.init)
+            res.qfim = E.qfim(J, psi).cpu().numpy()
+            res.state = psi
+        torch.cuda.synchronize()
+        res.done = time.perf_counter() - t0
+        out.append(res)
+    return out, time.perf_counter() - t0

@@ -323,3 +323,31 @@ def test_take_derivative_and_qfim_from_grad_list(pyqc):
     F1 = m.get_QFI()
     F2 = m.get_QFI(grad_list=qc.get_gradients(method="plain"))
     assert np.max(np.abs(F1 - F2)) < 1e-13
+
+
+def test_qfim_large_p_symmetric_update_path(pyqc):
+    """P >= 768 takes the real-GEMM (rocBLAS dgemm) route of pqs_qfim;
+    against the oracle QFIM and against the split-K Gram kernel route
+    (pqs_gram of J with itself, combined on the host)."""
+    import torch
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    n, p = 8, 48
+    qc = pyqc.PQC(n)
+    for l in T.clifford_HE_layers(p, n):
+        qc.add_layer(l)
+    P = qc.n_true_params
+    assert P == 768
+    th = np.random.default_rng(21).random(P) * 2 * np.pi
+    qc.state = qc.run(list(th))
+    psi, J = qc.jacobian()
+    F = E.qfim(J, psi).cpu().numpy()
+    og = OC.clifford_he(p, n)
+    ref = OS.run(og, th, n)
+    Fref = OM.qfim(OS.derivative_states(og, th, n), ref)
+    assert rel_fro(F, Fref) < 1e-9
+    G = E.gram(J, J).cpu().numpy()
+    b = E.gram(psi.reshape(1, -1), J).cpu().numpy().ravel()
+    F2 = 4 * np.real(G - np.outer(np.conj(b), b))
+    assert np.max(np.abs(F - F2)) < 1e-10
+    assert np.allclose(F, F.T, atol=0)

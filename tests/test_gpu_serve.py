@@ -1,0 +1,117 @@
+"""Serving loop on the GPU: each result is the same alone and in the mix,
+and matches the oracle; admission respects the HBM budget."""
+import numpy as np
+import pytest
+
+from oracle import circuits as OC
+from oracle import measures as OM
+from oracle import sim as OS
+from tests.helpers import infidelity, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sv():
+    from pyramaterised_amd import serve
+    return serve
+
+
+def _check_same(a, b):
+    assert a.kind == b.kind and a.rid == b.rid
+    if a.kind == "state_mw":
+        sa, sb = a.state.cpu().numpy().ravel(), b.state.cpu().numpy().ravel()
+        assert np.max(np.abs(sa - sb)) < 1e-12
+        assert abs(a.Q - b.Q) < 1e-12
+    else:
+        assert rel_fro(a.qfim, b.qfim) < 1e-12
+
+
+def test_alone_vs_mix_and_oracle(sv):
+    reqs = sv.make_workload(48, rate=0.0, seed=5, n_range=(6, 12), depth_range=(2, 6), qfim_frac=0.3)
+    srv = sv.Server(n_streams=3, max_batch=8)
+    mix = srv.run(reqs, realtime=False)
+    assert all(r.done is not None for r in mix)
+    assert max(srv.batch_sizes) > 1  # some classes were batched
+    alone, _ = sv.run_one_at_a_time(reqs)
+    for a, b in zip(mix, alone):
+        _check_same(a, b)
+    # oracle on a sample of each kind
+    checked = {"state_mw": 0, "qfim": 0}
+    for r, res in zip(reqs, mix):
+        if checked[r.kind] >= 3:
+            continue
+        og = OC.clifford_he(r.depth, r.n)
+        ref = OS.run(og, r.params, r.n)
+        if r.kind == "state_mw":
+            assert infidelity(res.state.cpu().numpy(), ref) < 1e-12
+            assert abs(res.Q - OM.single_Q(ref, r.n)) < 1e-12
+        else:
+            F = OM.qfim(OS.derivative_states(og, r.params, r.n), ref)
+            assert rel_fro(res.qfim, F) < 1e-9
+        checked[r.kind] += 1
+    assert checked == {"state_mw": 3, "qfim": 3}
+
+
+def test_realtime_arrivals_and_tight_budget(sv):
+    """Poisson arrivals; a budget that fits only about two of the largest
+    requests at once, so admission has to wait for memory."""
+    reqs = sv.make_workload(40, rate=400.0, seed=6, n_range=(8, 12), depth_range=(2, 4), qfim_frac=0.25)
+    probe = sv.Server(n_streams=1)
+    biggest = max(probe.request_bytes(r) for r in reqs)
+    srv = sv.Server(hbm_budget=2 * biggest + 1, n_streams=4, max_batch=16, keep_states=False)
+    res = srv.run(reqs, realtime=True)
+    assert srv.peak_in_use <= srv.budget
+    assert srv.in_use == 0
+    alone, _ = sv.run_one_at_a_time(reqs)
+    for a, b in zip(res, alone):
+        assert a.done >= a.arrival
+        if a.kind == "state_mw":
+            assert abs(a.Q - b.Q) < 1e-12
+        else:
+            assert rel_fro(a.qfim, b.qfim) < 1e-12
+    st = sv.stats(res, srv.wall)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_release_returns_budget(sv):
+    reqs = sv.make_workload(6, rate=0.0, seed=7, n_range=(6, 8), depth_range=(2, 3), qfim_frac=0.0)
+    srv = sv.Server(n_streams=2)
+    res = srv.run(reqs, realtime=False)
+    held = srv.in_use
+    assert held == sum(16 * (1 << r.n) for r in reqs)
+    for r in res:
+        srv.release(r.rid)
+    assert srv.in_use == 0
+
+
+def test_specialised_class_matches_alone(sv):
+    """Classes large enough for run-time specialised kernels: after the
+    background compile is loaded, results equal the interpreted path."""
+    import time
+    from pyramaterised_amd import engine as E
+    reqs = [r for r in sv.make_workload(30, rate=0.0, seed=8, n_range=(14, 14), depth_range=(2, 2),
+                                        qfim_frac=0.5)][:8]
+    srv = sv.Server(n_streams=2, specialise_qfim_from=14, specialise_state_from=14, keep_states=True)
+    c = srv.circuit(14, 2)
+    for which in ("plan", "gplan"):
+        c.specialise(which, srv._pool, 4)
+    t0 = time.time()
+    while c._pending and time.time() - t0 < 120:
+        c.poll()
+        time.sleep(0.05)
+    assert c._jit.get("plan") is not None and c._jit.get("gplan") is not None
+    assert c.best("plan").jitted and c.best("gplan").jitted
+    res = srv.run(reqs, realtime=False)
+    srv.close()
+    ref = {}
+    for r in reqs:
+        cc = sv._Circuit(r.n, r.depth, E.L.PQS_C128)  # interpreted only
+        ref[r.rid] = sv.run_one_at_a_time([r], circuits={(r.n, r.depth): cc})[0][0]
+    for a in res:
+        b = ref[a.rid]
+        if a.kind == "state_mw":
+            assert np.max(np.abs(a.state.cpu().numpy() - b.state.cpu().numpy())) < 1e-12
+            assert abs(a.Q - b.Q) < 1e-12
+        else:
+            assert rel_fro(a.qfim, b.qfim) < 1e-12
