@@ -8,7 +8,7 @@ traces, the QFIM and the sampling measures run in the HIP library
 ``lib/libpqs.so``; there is no CPU fallback.
 """
 from .circuit import *  # noqa: F401,F403
-from . import gates, measure, serve, templates, qobj  # noqa: F401
+from . import autograd, gates, measure, serve, templates, qobj  # noqa: F401
 from .qobj import Ket, PauliSum, Qobj, basis, bell_state, expect, qeye, sigmax, sigmay, sigmaz, tensor  # noqa: F401
 
 __version__ = "0.1.0"
