@@ -94,9 +94,18 @@ typedef struct pqs_plan_opts {
 enum {
   PQS_PLAN_LANES_ASCENDING = 1, /* thread bits in ascending order, not bank-aware */
   PQS_PLAN_NO_SOFT_REGS = 2,    /* do not pull CNOT / CZ qubits into registers   */
-  PQS_PLAN_JIT_PERSISTENT = 4   /* specialised kernels: persistent blocks that
+  PQS_PLAN_JIT_PERSISTENT = 4,  /* specialised kernels: persistent blocks that
                                    prefetch their next tile into registers (pays
                                    off only at one resident block per CU)         */
+  PQS_PLAN_ROLLING = 8          /* rolling layouts: each sweep writes the state
+                                   with the qubit -> bit-position map under which
+                                   the next sweep's tile is one contiguous block;
+                                   low_bits is then the least overlap kept between
+                                   consecutive tiles (contiguous write runs), not
+                                   a fixed set of qubits.  Runs go out of place
+                                   (ping-pong workspace of one state per
+                                   instance).  Ignored for plans with tangents
+                                   and for tiles below 10 qubits. */
 };
 
 /* One instance of a batched run. */
@@ -135,6 +144,11 @@ int pqs_plan_info(const pqs_plan *plan, int32_t *n_sweeps, int32_t *tile_bits,
  * A merged generator reports the index of its first term. */
 int pqs_plan_sweep(const pqs_plan *plan, int32_t s, int32_t *tile_qubits,
                    int32_t *n_tile, int32_t *op_index, int32_t *n_op);
+/* Bit positions of every qubit (n entries each) in the layout sweep s reads
+ * and in the layout it writes.  Position n-1-q is the standard (QuTiP) order;
+ * both are standard for every sweep of a plan without PQS_PLAN_ROLLING, and
+ * the first read and the last write are standard in every plan. */
+int pqs_plan_sweep_layout(const pqs_plan *plan, int32_t s, int32_t *read_pos, int32_t *write_pos);
 /* First sweep that executes the generator of tangent t (-1 if none). */
 int pqs_plan_tangent_sweep(const pqs_plan *plan, int32_t t, int32_t *sweep);
 

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libpqs.so")
 PQS_OK = 0
 PQS_C128, PQS_C64 = 0, 1
 # pqs_plan_opts.flags
-PQS_PLAN_LANES_ASCENDING, PQS_PLAN_NO_SOFT_REGS, PQS_PLAN_JIT_PERSISTENT = 1, 2, 4
+PQS_PLAN_LANES_ASCENDING, PQS_PLAN_NO_SOFT_REGS, PQS_PLAN_JIT_PERSISTENT, PQS_PLAN_ROLLING = 1, 2, 4, 8
 
 (OP_MAT1, OP_RX, OP_RY, OP_RZ, OP_MAT2, OP_CX, OP_CZ, OP_RXX, OP_RYY, OP_RZZ,
  OP_FSIM, OP_GEN, OP_GEN_FSIM) = range(13)
@@ -48,6 +48,7 @@ SIGNATURES = {
     "pqs_plan_destroy": (C.c_int, [vp]),
     "pqs_plan_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     "pqs_plan_sweep": (C.c_int, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+    "pqs_plan_sweep_layout": (C.c_int, [vp, i32, C.POINTER(i32), C.POINTER(i32)]),
     "pqs_plan_tangent_sweep": (C.c_int, [vp, i32, C.POINTER(i32)]),
     "pqs_run": (C.c_int, [vp, vp, C.POINTER(Instance), i64, vp, vp]),
     "pqs_run_batch": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, vp, i32, vp, vp, vp]),

@@ -885,8 +885,34 @@ __global__ __launch_bounds__(MAXT) void k_sweep_reg(const DevSweep sw, const Dev
   }
 
   // ---- scatter the tile
+  if (!sw.rolling) {
 #pragma unroll
-  for (int m = 0; m < NA; ++m) dst[gaddr(m)] = tile[swz(tid + m * nthr)];
+    for (int m = 0; m < NA; ++m) dst[gaddr(m)] = tile[swz(tid + m * nthr)];
+  } else {
+    // write layout: store index bits in ascending write position, so the
+    // lanes of a wave cover the lowest (contiguous) write positions
+    uint64_t wl = 0;
+    for (int j = 0; j < sw.n - sw.k; ++j)
+      if ((uint64_t(tile_id) >> j) & 1) wl |= uint64_t(1) << sw.cwpos[j];
+    int jl = 0;
+    for (int i = 0; i < kt; ++i)
+      if ((tid >> i) & 1) {
+        wl |= uint64_t(1) << sw.wpos[sw.sperm[i]];
+        jl |= 1 << sw.sperm[i];
+      }
+#pragma unroll
+    for (int m = 0; m < NA; ++m) {
+      uint64_t wa = wl;
+      int ja = jl;
+#pragma unroll
+      for (int b = 0; b < pqs::kRegBits; ++b)
+        if (m & (1 << b)) {
+          wa |= uint64_t(1) << sw.wpos[sw.sperm[kt + b]];
+          ja |= 1 << sw.sperm[kt + b];
+        }
+      dst[wa] = tile[swz(ja)];
+    }
+  }
 }
 
 // --------------------------------------------------------------- reductions
@@ -1547,6 +1573,37 @@ int pqs_plan_destroy(pqs_plan *p) {
   return PQS_OK;
 }
 
+// Rolling-layout plans change the bit layout from sweep to sweep, so every
+// instance runs out of place: its second buffer is taken from the
+// workspace, [alt states | instance table] (returned: the table's offset).
+// An instance whose input is its own output buffer, while the first sweep
+// would write that buffer, first copies the input to its second buffer.
+static int rolling_setup(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, hipStream_t st, size_t *tab_off) {
+  *tab_off = 0;
+  if (!p->rolling) return ws_reserve(ctx, h.size() * sizeof(InstDev));
+  const size_t sb = (size_t(1) << p->n) * amp_bytes(p->dtype);
+  const size_t alt_b = h.size() * sb;
+  *tab_off = (alt_b + 255) & ~size_t(255);
+  int rc = ws_reserve(ctx, *tab_off + h.size() * sizeof(InstDev));
+  if (rc) return rc;
+  const int S = int(p->dev_sweeps.size());
+  const bool first_writes_state = ((S - 1) & 1) == 0;
+  for (size_t i = 0; i < h.size(); ++i) {
+    InstDev &d = h[i];
+    d.alt = (char *)ctx->ws + i * sb;
+    d.pingpong = 1;
+    if (d.init == 0 && (d.src == nullptr || d.src == d.state)) {
+      if (first_writes_state) {
+        HIPCHECK(hipMemcpyAsync(d.alt, d.state, sb, hipMemcpyDeviceToDevice, st));
+        d.src = d.alt;
+      } else {
+        d.src = d.state;
+      }
+    }
+  }
+  return PQS_OK;
+}
+
 int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst, const void *init_tab,
             void *stream) {
   if (!ctx || !p) return fail(PQS_ERR_ARG, "NULL ctx or plan");
@@ -1574,13 +1631,14 @@ int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst
   }
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
-  size_t tb = h.size() * sizeof(InstDev);
-  rc = ws_reserve(ctx, tb);
+  size_t tab_off = 0;
+  rc = rolling_setup(ctx, p, h, st, &tab_off);
   if (rc) return rc;
-  rc = upload(ctx, ctx->ws, h.data(), tb, st);
+  const InstDev *d_tab = (const InstDev *)((char *)ctx->ws + tab_off);
+  rc = upload(ctx, (void *)d_tab, h.data(), h.size() * sizeof(InstDev), st);
   if (rc) return rc;
   for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
-    rc = launch_sweep(ctx, p, s, (const InstDev *)ctx->ws, n_inst, init_tab, st);
+    rc = launch_sweep(ctx, p, s, d_tab, n_inst, init_tab, st);
     if (rc) return rc;
   }
   return PQS_OK;
@@ -1614,13 +1672,14 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride,
   }
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
-  const size_t tb = h.size() * sizeof(InstDev);
-  rc = ws_reserve(ctx, tb);
+  size_t tab_off = 0;
+  rc = rolling_setup(ctx, p, h, st, &tab_off);
   if (rc) return rc;
-  rc = upload(ctx, ctx->ws, h.data(), tb, st);
+  const InstDev *d_tab = (const InstDev *)((char *)ctx->ws + tab_off);
+  rc = upload(ctx, (void *)d_tab, h.data(), h.size() * sizeof(InstDev), st);
   if (rc) return rc;
   for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
-    rc = launch_sweep(ctx, p, s, (const InstDev *)ctx->ws, batch, init_tab, st);
+    rc = launch_sweep(ctx, p, s, d_tab, batch, init_tab, st);
     if (rc) return rc;
   }
   return PQS_OK;
@@ -1633,6 +1692,7 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
   if (init < 0 || init > 2) return fail(PQS_ERR_ARG, "init must be 0, 1 or 2");
   if (init == 0 && !src) return fail(PQS_ERR_ARG, "init 0 needs a source state");
   if (init == 2 && !init_tab) return fail(PQS_ERR_ARG, "product init needs an init table");
+  if (p->rolling) return fail(PQS_ERR_UNSUPPORTED, "rolling-layout plan in pqs_jacobian");
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
   const int S = int(p->dev_sweeps.size());

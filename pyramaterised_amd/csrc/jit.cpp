@@ -388,7 +388,30 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       for (int v = 0; v < NA_(); ++v) o << "    tile[pt ^ " << kv[v] << "] = a[" << v << "];\n";
       o << "    __syncthreads();\n  }\n";
     }
-    for (int m = 0; m < NA_(); ++m) o << "  dst[dl | " << hp(m) << "ull] = tile[swz(tid + " << m * nthr << ")];\n";
+    if (!ds.rolling) {
+      for (int m = 0; m < NA_(); ++m) o << "  dst[dl | " << hp(m) << "ull] = tile[swz(tid + " << m * nthr << ")];\n";
+    } else {
+      // write layout: store index bit i is tile bit sperm[i], at write
+      // position wpos[sperm[i]] (ascending), complement bit j at cwpos[j]
+      o << "  {\n    const u64 wtid = u64(t) & " << (ds.n_tiles - 1) << "ull;\n";
+      o << "    const u64 wdl = 0ull";
+      for (int j = 0; j < p.n - k; ++j) o << " | (((wtid >> " << j << ") & 1ull) << " << ds.cwpos[j] << ")";
+      for (int i = 0; i < kt; ++i) o << " | (u64((tid >> " << i << ") & 1) << " << ds.wpos[ds.sperm[i]] << ")";
+      o << ";\n    const int jdl = 0";
+      for (int i = 0; i < kt; ++i) o << " | (((tid >> " << i << ") & 1) << " << ds.sperm[i] << ")";
+      o << ";\n";
+      for (int m = 0; m < NA_(); ++m) {
+        unsigned long long cm = 0;
+        int jm = 0;
+        for (int b = 0; b < kRegBits; ++b)
+          if (m & (1 << b)) {
+            cm |= 1ull << ds.wpos[ds.sperm[kt + b]];
+            jm |= 1 << ds.sperm[kt + b];
+          }
+        o << "    dst[wdl | " << cm << "ull] = tile[swz(jdl | " << jm << ")];\n";
+      }
+      o << "  }\n";
+    }
     o << "  }\n}\n";
   }
 }

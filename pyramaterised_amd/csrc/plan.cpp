@@ -185,42 +185,99 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
   std::vector<int> remaining(p.exec.size());
   for (size_t i = 0; i < remaining.size(); ++i) remaining[i] = int(i);
 
+  // Rolling layouts: the qubits kept from one tile to the next (at least
+  // `low` of them) become the lowest bit positions of the layout in between,
+  // so both the contiguous tile reads and the write runs are preserved while
+  // the tile moves freely over the qubits.
+  p.rolling = (o.flags & PQS_PLAN_ROLLING) && n > k && k >= kRegBits + 6 && p.n_tangents == 0;
+
+  // One sweep: greedy over the remaining ops with `want` in the tile first;
+  // if `want` leaves no room for the next op, without it.
+  auto plan_one = [&](qmask want, qmask &T, std::vector<int> &run, std::vector<int> &rest) {
+    run.clear();
+    for (int attempt = 0; attempt < 2 && run.empty(); ++attempt) {
+      T = (n <= k) ? all : (attempt == 0 ? want : 0);
+      run.clear();
+      rest.clear();
+      qmask blocked_any = 0, blocked_nd = 0;
+      int n_angle = 0;
+      for (int ei : remaining) {
+        const ExecOp &e = p.exec[ei];
+        qmask touched = e.active | e.diag;
+        bool dep = (touched & blocked_nd) || (e.active & blocked_any);
+        bool ok = false;
+        if (n_angle >= kMaxSweepOps) dep = true;  // the sweep's LDS op table is full
+        if (!dep) {
+          qmask need = e.active & ~T;
+          if (popc(T | need) <= k) {
+            T |= need;
+            ok = true;
+            ++n_angle;
+          }
+        }
+        if (ok) {
+          run.push_back(ei);
+        } else {
+          rest.push_back(ei);
+          blocked_any |= touched;
+          blocked_nd |= e.active;
+        }
+      }
+      if (remaining.empty()) break;
+    }
+  };
+  // qubits of `from` ordered by their first non-diagonal use in `ops_left`
+  auto by_next_use = [&](qmask from, const std::vector<int> &ops_left) {
+    std::vector<int> order;
+    qmask seen = 0;
+    for (int ei : ops_left) {
+      qmask a = p.exec[ei].active & from & ~seen;
+      for (int q = 0; q < n; ++q)
+        if ((a >> q) & 1) order.push_back(q);
+      seen |= a;
+    }
+    for (int q = 0; q < n; ++q)
+      if (((from & ~seen) >> q) & 1) order.push_back(q);
+    return order;
+  };
+
+  qmask prevT = 0;
   while (!remaining.empty() || p.sweeps.empty()) {
     qmask T = 0;
     std::vector<int> run, rest;
-    // First try with the forced low positions in the tile (coalesced
-    // gathers); if they leave no room for the next op, drop them.
-    for (int attempt = 0; attempt < 2 && run.empty(); ++attempt) {
-    T = (n <= k) ? all : (attempt == 0 ? forced : 0);
-    run.clear();
-    rest.clear();
-    qmask blocked_any = 0, blocked_nd = 0;
-    int n_angle = 0;
-    for (int ei : remaining) {
-      const ExecOp &e = p.exec[ei];
-      qmask touched = e.active | e.diag;
-      bool dep = (touched & blocked_nd) || (e.active & blocked_any);
-      bool ok = false;
-      if (n_angle >= kMaxSweepOps) dep = true;  // the sweep's LDS op table is full
-      if (!dep) {
-        qmask need = e.active & ~T;
-        if (popc(T | need) <= k) {
-          T |= need;
-          ok = true;
-          ++n_angle;
-        }
-      }
-      if (ok) {
-        run.push_back(ei);
-      } else {
-        rest.push_back(ei);
-        blocked_any |= touched;
-        blocked_nd |= e.active;
+    qmask want = forced;
+    if (p.rolling && !p.sweeps.empty()) {
+      want = 0;  // carry the `low` previous-tile qubits needed soonest
+      for (int q : by_next_use(prevT, remaining)) {
+        if (popc(want) >= low) break;
+        want |= bitq(q);
       }
     }
-    if (remaining.empty()) break;
-    }
+    plan_one(want, T, run, rest);
     if (run.empty() && !remaining.empty()) return fail(PQS_ERR_UNSUPPORTED, "planner made no progress");
+    if (p.rolling && rest.empty() && (T & forced) != forced && popc(want | forced) <= k) {
+      // the last sweep writes the standard layout: keep its lowest qubits in
+      // the tile (a shorter last sweep and one more are cheaper than a
+      // scattered final write)
+      qmask T2;
+      std::vector<int> run2, rest2;
+      plan_one(want | forced, T2, run2, rest2);
+      if (!run2.empty() && (T2 & forced) == forced) {
+        T = T2;
+        run.swap(run2);
+        rest.swap(rest2);
+      }
+    }
+    if (p.rolling) {
+      // fill: the standard lowest qubits if this is the last sweep, then the
+      // previous tile's qubits (longer write runs), then the lowest positions
+      if (rest.empty())
+        for (int b = 0; b < n && popc(T) < k; ++b) T |= bitq(n - 1 - b) & forced;
+      for (int q : by_next_use(prevT & ~T, rest)) {
+        if (popc(T) >= k) break;
+        T |= bitq(q);
+      }
+    }
     // Fill the tile to k qubits with the lowest free positions.
     for (int b = 0; b < n && popc(T) < k; ++b)
       if (!(T & bitq(n - 1 - b))) T |= bitq(n - 1 - b);
@@ -229,8 +286,44 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
       if (T & bitq(n - 1 - b)) sw.tile_qubits.push_back(n - 1 - b);
     sw.ops = run;
     p.sweeps.push_back(sw);
+    prevT = T;
     remaining.swap(rest);
     if (remaining.empty()) break;
+  }
+
+  // Layouts: L_0 and L_S standard (pos = n-1-q); in between, L_s holds the
+  // qubits kept from tile s-1 lowest, then the rest of tile s, then the
+  // other qubits in qubit order.
+  {
+    const int S = int(p.sweeps.size());
+    std::vector<std::vector<int>> L(S + 1, std::vector<int>(n));
+    for (int q = 0; q < n; ++q) L[0][q] = L[S][q] = n - 1 - q;
+    for (int s = 1; s < S; ++s) {
+      if (!p.rolling) {
+        L[s] = L[0];
+        continue;
+      }
+      qmask Tp = 0, Tc = 0;
+      for (int q : p.sweeps[s - 1].tile_qubits) Tp |= bitq(q);
+      for (int q : p.sweeps[s].tile_qubits) Tc |= bitq(q);
+      std::vector<int> keep;
+      for (int q = 0; q < n; ++q)
+        if ((Tp & Tc) & bitq(q)) keep.push_back(q);
+      std::sort(keep.begin(), keep.end(), [&](int a, int b) { return L[s - 1][a] < L[s - 1][b]; });
+      int pos = 0;
+      for (int q : keep) L[s][q] = pos++;
+      for (int q = 0; q < n; ++q)
+        if ((Tc & ~Tp) & bitq(q)) L[s][q] = pos++;
+      for (int q = 0; q < n; ++q)
+        if (!(Tc & bitq(q))) L[s][q] = pos++;
+    }
+    for (int s = 0; s < S; ++s) {
+      Sweep &sw = p.sweeps[s];
+      sw.rpos = L[s];
+      sw.wpos = L[s + 1];
+      std::sort(sw.tile_qubits.begin(), sw.tile_qubits.end(),
+                [&](int a, int b) { return sw.rpos[a] < sw.rpos[b]; });
+    }
   }
 
   p.tangent_sweep.assign(p.n_tangents, -1);
@@ -252,13 +345,20 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     DevSweep ds{};
     int tile_index[kMaxQubits];
     for (int q = 0; q < n; ++q) tile_index[q] = -1;
-    std::vector<int> tpos, cpos;
+    std::vector<int> tpos, cpos, cq;
     for (size_t i = 0; i < sw.tile_qubits.size(); ++i) {
       tile_index[sw.tile_qubits[i]] = int(i);
-      tpos.push_back(n - 1 - sw.tile_qubits[i]);
+      tpos.push_back(sw.rpos[sw.tile_qubits[i]]);
     }
-    for (int b = 0; b < n; ++b)
-      if (std::find(tpos.begin(), tpos.end(), b) == tpos.end()) cpos.push_back(b);
+    {
+      std::vector<int> qat(n);
+      for (int q = 0; q < n; ++q) qat[sw.rpos[q]] = q;
+      for (int b = 0; b < n; ++b)
+        if (tile_index[qat[b]] < 0) {
+          cpos.push_back(b);
+          cq.push_back(qat[b]);
+        }
+    }
     auto runs = [](const std::vector<int> &pos, int32_t *rp, int32_t *rl, int32_t *rs) {
       int nr = 0;
       for (size_t i = 0; i < pos.size();) {
@@ -276,6 +376,16 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     ds.nruns = runs(tpos, ds.run_pos, ds.run_len, ds.run_shift);
     ds.cnruns = runs(cpos, ds.crun_pos, ds.crun_len, ds.crun_shift);
     ds.n_tiles = int64_t(1) << cpos.size();
+    ds.rolling = p.rolling ? 1 : 0;
+    ds.n = n;
+    {
+      for (int i = 0; i < ds.k; ++i) ds.wpos[i] = sw.wpos[sw.tile_qubits[i]];
+      for (size_t j = 0; j < cq.size(); ++j) ds.cwpos[j] = sw.wpos[cq[j]];
+      std::vector<int> perm(ds.k);
+      for (int i = 0; i < ds.k; ++i) perm[i] = i;
+      std::sort(perm.begin(), perm.end(), [&](int a, int b) { return ds.wpos[a] < ds.wpos[b]; });
+      for (int i = 0; i < ds.k; ++i) ds.sperm[i] = perm[i];
+    }
 
     // Phases: LDS ops (two-qubit dense, generators) run alone and order
     // everything around them; between them, greedy register phases of at
@@ -410,9 +520,9 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         d.scale = so.scale;
         d.offset = so.offset;
         std::memcpy(d.m, so.m, sizeof d.m);
-        d.g0 = n - 1 - so.q0;
+        d.g0 = sw.rpos[so.q0];
         d.t0 = tile_index[so.q0];
-        d.g1 = so.q1 >= 0 ? n - 1 - so.q1 : -1;
+        d.g1 = so.q1 >= 0 ? sw.rpos[so.q1] : -1;
         d.t1 = so.q1 >= 0 ? tile_index[so.q1] : -1;
         d.r0 = d.t0 >= 0 ? reg_index[d.t0] : -1;
         d.r1 = d.t1 >= 0 ? reg_index[d.t1] : -1;
@@ -443,7 +553,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
               if (x) dt.xm |= 1u << tile_index[q];  // active qubits are in the tile
               if (z) {
                 if (tile_index[q] >= 0) dt.zm_tile |= 1u << tile_index[q];
-                else dt.zm_glob |= uint64_t(1) << (n - 1 - q);
+                else dt.zm_glob |= uint64_t(1) << sw.rpos[q];
               }
             }
             double cr = g.m[0], ci = g.m[1];  // coef * i^ny
@@ -514,6 +624,17 @@ int pqs_plan_info(const pqs_plan *p, int32_t *n_sweeps, int32_t *tile_bits,
   if (tile_bits) *tile_bits = p->tile_bits;
   if (n_exec_ops) *n_exec_ops = int32_t(p->exec.size());
   if (n_tangents) *n_tangents = p->n_tangents;
+  return PQS_OK;
+}
+
+int pqs_plan_sweep_layout(const pqs_plan *p, int32_t s, int32_t *read_pos, int32_t *write_pos) {
+  if (!p) return pqs::fail(PQS_ERR_ARG, "plan is NULL");
+  if (s < 0 || s >= int32_t(p->sweeps.size())) return pqs::fail(PQS_ERR_ARG, "sweep index out of range");
+  const pqs::Sweep &sw = p->sweeps[s];
+  for (int q = 0; q < p->n; ++q) {
+    if (read_pos) read_pos[q] = sw.rpos[q];
+    if (write_pos) write_pos[q] = sw.wpos[q];
+  }
   return PQS_OK;
 }
 

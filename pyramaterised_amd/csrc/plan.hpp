@@ -30,8 +30,9 @@ constexpr int kPauliSum = 100;
 constexpr int kGen1 = 101;
 
 struct Sweep {
-  std::vector<int> tile_qubits; // ascending bit position (pos = N-1-q)
+  std::vector<int> tile_qubits; // ascending bit position in the read layout
   std::vector<int> ops;         // ExecOp indices in execution order
+  std::vector<int> rpos, wpos;  // per qubit: bit position read / written
 };
 
 // ---- device image -------------------------------------------------------
@@ -80,6 +81,14 @@ struct DevSweep {
   int64_t n_tiles;
   int32_t phase_begin, phase_end;  // register phases (reg executor only)
   int32_t n_coef;                  // 1: the sweep has LDS phases (dense 2q ops, generators)
+  // Write layout (rolling plans): the tile is stored through a permutation
+  // of its bits, store index bit i <-> tile bit sperm[i], ordered by
+  // ascending write position wpos[sperm[i]]; complement (tile id) bit j is
+  // written at cwpos[j].  rolling = 0: write layout = read layout.
+  int32_t rolling;
+  int32_t n;  // qubits of the state
+  int32_t sperm[kMaxTileBits], wpos[kMaxTileBits];
+  int32_t cwpos[kMaxQubits];
 };
 
 std::string plan_error(); // last planner error (thread local)
@@ -93,6 +102,7 @@ struct pqs_plan {
   int block = 0;
   int dtype = 0;
   int flags = 0;
+  bool rolling = false;  // rolling layouts in effect (PQS_PLAN_ROLLING honoured)
   int n_tangents = 0;
   std::vector<pqs_op> src_ops;
   std::vector<pqs::ExecOp> exec;

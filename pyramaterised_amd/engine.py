@@ -144,16 +144,18 @@ def tile_candidates(n, dtype_c):
     """Tile shapes (tile bits, forced low bits, flags) the tuner times.
 
     c128: 12-bit tiles (64 KiB of LDS, two resident blocks per CU, so one
-    block's LDS phases overlap the other's HBM traffic) with 128 B or 256 B
-    contiguous runs, and 13-bit tiles (one block per CU) in persistent form.
-    c64: 13- and 12-bit tiles (64 / 32 KiB).  Fewer forced low bits leave
-    more tile room for the circuit (fewer sweeps) at some cost in access
-    contiguity; which wins depends on the circuit and the batch.
+    block's LDS phases overlap the other's HBM traffic) and 11-bit tiles
+    (five blocks per CU) with rolling layouts (the tile moves freely over
+    the qubits; out of place) and 128 B or 256 B write runs, or fixed low
+    qubits (in place).  c64: 11- to 13-bit tiles (16-64 KiB).  Which wins
+    depends on the circuit and the batch: deep sweeps favour more resident
+    blocks, shallow HBM-bound ones larger tiles.
     """
+    R = L.PQS_PLAN_ROLLING
     if dtype_c == L.PQS_C128:
-        c = [(12, 3, 0), (12, 4, 0), (13, 3, L.PQS_PLAN_JIT_PERSISTENT)]
+        c = [(12, 4, R), (12, 3, R), (11, 3, R), (12, 4, 0)]
     else:
-        c = [(13, 4, 0), (12, 4, 0), (13, 3, 0)]
+        c = [(12, 4, R), (13, 4, R), (11, 4, R), (13, 4, 0)]
     out = []
     for k, low, fl in c:
         k = min(k, n)
@@ -249,6 +251,7 @@ class Plan:
         ns, tb, ne, nt = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         check(L.lib().pqs_plan_info(self.handle, C.byref(ns), C.byref(tb), C.byref(ne), C.byref(nt)))
         self.n_sweeps, self.tile_bits, self.n_exec, self.n_tangents = ns.value, tb.value, ne.value, nt.value
+        self.jitted = False
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -268,6 +271,20 @@ class Plan:
             check(L.lib().pqs_plan_sweep(self.handle, s, tq, C.byref(nt), oi, C.byref(no)))
             out.append((list(tq[:nt.value]), list(oi[:no.value])))
         return out
+
+    def layouts(self):
+        """[(read_pos, write_pos)] per sweep: bit position of every qubit in
+        the layout the sweep reads and the one it writes."""
+        out = []
+        for s in range(self.n_sweeps):
+            r, w = (C.c_int32 * self.n)(), (C.c_int32 * self.n)()
+            check(L.lib().pqs_plan_sweep_layout(self.handle, s, r, w))
+            out.append((list(r), list(w)))
+        return out
+
+    @property
+    def rolling(self):
+        return any(r != w for r, w in self.layouts())
 
     def jit(self, load=True, workers=None):
         """Specialise the plan's sweeps into straight-line kernels (hiprtc);

@@ -147,3 +147,69 @@ def test_jit_tile_shapes_persistent_and_tuner(pyqc, dtype, tol):
     st = qc.run_batch(params).cpu().numpy()
     for s in range(S):
         assert infidelity(st[s], refs[s]) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 1e-5)])
+@pytest.mark.parametrize("jit", [True, False])
+def test_rolling_layouts_match_oracle(pyqc, dtype, tol, jit):
+    """Rolling-layout plans (out of place, layout changes between sweeps)
+    in batches and single runs, specialised and interpreted."""
+    from pyramaterised_amd import _lib as L
+    from pyramaterised_amd import templates as T
+    n, p, S = 20, 8, 3
+    qc = _build(pyqc, n, T.ry_cz_brickwork_layers(p, n), jit, dtype=dtype, tile_bits=12, low_bits=3,
+                flags=L.PQS_PLAN_ROLLING)
+    plan = qc.plan(batch=S)
+    assert plan.rolling and plan.n_sweeps > 2 and plan.jitted == jit
+    og = OC.ry_cz_brickwork(p, n)
+    params = np.random.default_rng(40).random((S, n * p)) * 2 * np.pi
+    st = qc.run_batch(params).cpu().numpy()
+    for s in range(S):
+        assert infidelity(st[s], OS.run(og, params[s], n)) < tol
+    one = qc.run(list(params[1])).full().ravel()
+    assert np.max(np.abs(one - st[1])) < (1e-13 if dtype == "complex128" else 1e-6)
+
+
+def test_rolling_all_reg_kinds_and_inplace_input(pyqc):
+    """All register op kinds under rolling layouts; a vector input state;
+    and the C ABI's in-place form (init 0, no source: the state buffer is
+    the input) for plans with an odd and an even number of sweeps."""
+    import ctypes as C
+    import torch
+    from pyramaterised_amd import _lib as L
+    from pyramaterised_amd import engine as E
+    rng = np.random.default_rng(41)
+    n = 14
+    kinds1 = ["R_x", "R_y", "R_z", "H", "T", "S", "X"]
+    kinds2 = ["CNOT", "CZ", "R_zz"]
+    parities = set()
+    for depth in (2, 3, 7, 8):
+        srng = np.random.default_rng(100 + depth)
+        spec = []
+        for _ in range(depth):
+            for q in range(n):
+                spec.append((kinds1[srng.integers(len(kinds1))], q))
+            for _ in range(n):
+                a, b = (int(x) for x in srng.choice(n, 2, replace=False))
+                spec.append((kinds2[srng.integers(len(kinds2))], (a, b)))
+        pk = [getattr(pyqc, nm)(list(q) if isinstance(q, tuple) else q, n) for nm, q in spec]
+        og = [getattr(OC, nm)(q, n) for nm, q in spec]
+        qc = _build(pyqc, n, [pk], True, tile_bits=10, low_bits=3, flags=L.PQS_PLAN_ROLLING)
+        plan = qc.plan()
+        assert plan.rolling and plan.jitted
+        parities.add(plan.n_sweeps % 2)
+        th = rng.random(qc.n_true_params) * 2 * np.pi
+        v = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+        v /= np.linalg.norm(v)
+        st = E.run(plan, th, E.Init.from_vector(v))[0].cpu().numpy()
+        ref = OS.run(og, th, n, v)
+        assert infidelity(st, ref) < 1e-12
+        # in place through the C ABI
+        buf = torch.as_tensor(v, device="cuda").reshape(1, -1).contiguous()
+        prm = torch.as_tensor(th, device="cuda")
+        ctx = E.context()
+        E.check(L.lib().pqs_run_batch(ctx.handle, plan.handle, C.c_void_p(buf.data_ptr()), 2 ** n, 1,
+                                      C.c_void_p(prm.data_ptr()), len(th), None, 0, None, None, E.stream_ptr()))
+        got = buf.cpu().numpy().ravel()
+        assert np.max(np.abs(got - ref)) < 1e-12
+    assert parities == {0, 1}

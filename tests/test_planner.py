@@ -221,3 +221,70 @@ def test_specialised_kernels_compile_without_gpu(pyqc, tmp_path, monkeypatch):
     srcs = plan.jit_sweep_sources()
     assert plan.n_sweeps > 1 and len(srcs) == plan.n_sweeps
     plan.jit(load=False, workers=1)
+
+
+def _layout_checks(plan, n, k, low, ops):
+    lay = plan.layouts()
+    sweeps = plan.sweeps()
+    std = [n - 1 - q for q in range(n)]
+    assert lay[0][0] == std and lay[-1][1] == std
+    for s, (r, w) in enumerate(lay):
+        assert sorted(r) == list(range(n)) and sorted(w) == list(range(n))
+        if s + 1 < len(lay):
+            assert w == lay[s + 1][0]
+        tq = sweeps[s][0]
+        assert len(tq) == k
+        if s > 0:
+            assert sorted(r[q] for q in tq) == list(range(k))  # contiguous tile read
+        # the lowest `low` write positions hold tile qubits (contiguous write runs)
+        low_w = sorted(w[q] for q in tq)[:low]
+        assert low_w == list(range(low)), (s, low_w)
+        for i in sweeps[s][1]:
+            assert active_qubits(ops[i]) <= set(tq)
+
+
+@pytest.mark.parametrize("n,tile,low,seed", [(14, 10, 3, 5), (16, 12, 4, 6), (20, 12, 3, 7), (24, 13, 3, 8)])
+def test_rolling_layouts_random_circuits(pyqc, n, tile, low, seed):
+    """Rolling layouts: every layout a permutation, consecutive sweeps agree
+    on the layout between them, standard at both ends, contiguous tile
+    reads, at least `low` contiguous write bits; op order equivalent."""
+    from pyramaterised_amd import engine as E
+    rng = np.random.default_rng(seed)
+    spec = [s for s in random_spec(n, 4, rng) if s[0] != "shared_Rx"]
+    qc = pyqc.PQC(n)
+    qc.add_layer(build_pkg(spec, n, pyqc))
+    ops = qc.ops()
+    plan = E.Plan(n, ops, tile_bits=tile, low_bits=low, flags=L.PQS_PLAN_ROLLING)
+    assert plan.n_sweeps > 1 and plan.rolling
+    _layout_checks(plan, n, tile, low, ops)
+    order = [i for _, idx in plan.sweeps() for i in idx]
+    assert sorted(order) == list(range(len(ops)))
+    if n <= 16:
+        params = rng.random(qc.n_true_params) * 2 * np.pi
+        a = simulate_ops(ops, range(len(ops)), n, params)
+        b = simulate_ops(ops, order, n, params)
+        assert np.max(np.abs(a - b)) < 1e-12
+
+
+def test_rolling_layouts_cut_sweeps_at_28q(pyqc):
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    qc = pyqc.PQC(28)
+    for l in T.ry_cz_brickwork_layers(30, 28):
+        qc.add_layer(l)
+    ops = qc.ops()
+    fixed = E.Plan(28, ops, tile_bits=12, low_bits=4)
+    roll = E.Plan(28, ops, tile_bits=12, low_bits=4, flags=L.PQS_PLAN_ROLLING)
+    assert not fixed.rolling and roll.rolling
+    assert roll.n_sweeps < 0.6 * fixed.n_sweeps
+    _layout_checks(roll, 28, 12, 4, ops)
+
+
+def test_rolling_ignored_with_tangents(pyqc):
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    qc = pyqc.PQC(16)
+    for l in T.clifford_HE_layers(4, 16):
+        qc.add_layer(l)
+    plan = E.Plan(16, qc.ops(True), tile_bits=12, low_bits=3, flags=L.PQS_PLAN_ROLLING)
+    assert not plan.rolling
