@@ -280,7 +280,8 @@ def leg_serving(args):
     # state (plans and specialised kernels already built)
     sat = SV.Server(keep_states=False)
     sat.circuits = srv.circuits
-    res_sat = sat.run(reqs, realtime=False)
+    at0 = [SV.Request(r.rid, r.n, r.depth, r.kind, r.params, 0.0) for r in reqs]
+    res_sat = sat.run(at0, realtime=False)
     st_sat = SV.stats(res_sat, sat.wall)
     sat.close()
     alone, wall = SV.run_one_at_a_time(reqs, circuits=srv.circuits)
