@@ -97,6 +97,10 @@ enum {
   PQS_PLAN_JIT_PERSISTENT = 4,  /* specialised kernels: persistent blocks that
                                    prefetch their next tile into registers (pays
                                    off only at one resident block per CU)         */
+  PQS_PLAN_NO_WAVE_LOCAL = 16,  /* specialised kernels: end every register phase
+                                   with a workgroup barrier (default: phases that
+                                   keep the same wave bits exchange through LDS
+                                   within each wave, without one) */
   PQS_PLAN_ROLLING = 8          /* rolling layouts: each sweep writes the state
                                    with the qubit -> bit-position map under which
                                    the next sweep's tile is one contiguous block;

@@ -386,7 +386,21 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       }
       o << "    rscale(a, R(fs));\n";
       for (int v = 0; v < NA_(); ++v) o << "    tile[pt ^ " << kv[v] << "] = a[" << v << "];\n";
-      o << "    __syncthreads();\n  }\n";
+      // the next phase keeps the wave bits (thread bits 6..): its LDS reads
+      // touch only this wave's amplitudes, which this wave's own earlier
+      // LDS writes (executed in order) already hold
+      bool wave_local = false;
+      if (ph + 1 < ds.phase_end && !(p.flags & PQS_PLAN_NO_WAVE_LOCAL)) {
+        const DevPhase &Q = p.dev_phases[ph + 1];
+        wave_local = !Q.lds && Q.ntb == P.ntb;
+        for (int i = 6; wave_local && i < P.ntb; ++i) wave_local = Q.tb[i] == P.tb[i];
+      }
+      if (wave_local)
+        o << "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\");\n"
+             "    __builtin_amdgcn_wave_barrier();\n"
+             "    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");\n  }\n";
+      else
+        o << "    __syncthreads();\n  }\n";
     }
     if (!ds.rolling) {
       for (int m = 0; m < NA_(); ++m) o << "  dst[dl | " << hp(m) << "ull] = tile[swz(tid + " << m * nthr << ")];\n";

@@ -455,7 +455,49 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     ds.phase_begin = int(p.dev_phases.size());
     int n_coef = 0;
     std::vector<int> exec_order;
+    // Wave bits: thread bits 6.. select the wave of a workgroup.  A run of
+    // register phases that keeps the same tile bits there (none of them a
+    // register bit) only permutes amplitudes within each wave, so the
+    // specialised kernels exchange through LDS without a workgroup barrier
+    // inside the run.  Per run, the wave bits are the tile bits needed
+    // latest.
+    const int nwb = std::max(0, ds.k - kRegBits - 6);
+    std::vector<int> phase_w(phases.size(), 0);  // tile-bit mask of the wave bits
+    if (p.reg_exec && nwb > 0 && !(p.flags & PQS_PLAN_NO_WAVE_LOCAL)) {
+      std::vector<int> rt(phases.size(), 0);
+      for (size_t i = 0; i < phases.size(); ++i)
+        for (int q = 0; q < n; ++q)
+          if (((phases[i].regs >> q) & 1) && tile_index[q] >= 0) rt[i] |= 1 << tile_index[q];
+      size_t i = 0;
+      while (i < phases.size()) {
+        if (phases[i].lds) {
+          ++i;
+          continue;
+        }
+        std::vector<std::pair<size_t, int>> cand;  // (next use, tile bit)
+        for (int t = 0; t < ds.k; ++t) {
+          if ((rt[i] >> t) & 1) continue;
+          size_t nu = phases.size();
+          for (size_t j = i; j < phases.size() && !phases[j].lds; ++j)
+            if ((rt[j] >> t) & 1) {
+              nu = j;
+              break;
+            }
+          cand.push_back({nu, t});
+        }
+        std::stable_sort(cand.begin(), cand.end(), [](const std::pair<size_t, int> &a, const std::pair<size_t, int> &b) {
+          return a.first > b.first;
+        });
+        int W = 0;
+        for (int c = 0; c < nwb && c < int(cand.size()); ++c) W |= 1 << cand[size_t(c)].second;
+        size_t j = i;
+        while (j < phases.size() && !phases[j].lds && !(rt[j] & W)) phase_w[j++] = W;
+        i = j;
+      }
+    }
+    size_t phase_no = 0;
     for (const PhaseSpec &ph : phases) {
+      const int wmask = phase_w[phase_no++];
       DevPhase dp{};
       dp.op_begin = int(p.dev_ops.size());
       dp.lds = ph.lds ? 1 : 0;
@@ -463,6 +505,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
       for (int t = 0; t < kMaxTileBits; ++t) reg_index[t] = -1;
       if (p.reg_exec && !ph.lds) {
         // register bits: the phase's qubits, then the lowest other tile bits
+        // (not wave bits)
         int nr = 0;
         for (int q = 0; q < n; ++q)
           if ((ph.regs >> q) & 1) {
@@ -472,18 +515,18 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
             dp.rb[nr++] = tile_index[q];
           }
         for (int t = 0; t < ds.k && nr < kRegBits; ++t)
-          if (reg_index[t] < 0) {
+          if (reg_index[t] < 0 && !((wmask >> t) & 1)) {
             reg_index[t] = nr;
             dp.rb[nr++] = t;
           }
         // Thread bits; the first ones become lane bits 0, 1, 2, ... and are
         // ordered so that their images under the LDS swizzle (tile bit t
         // lands on bank-group bit t mod 4) differ, spreading each group of
-        // lanes over distinct banks.
+        // lanes over distinct banks; the wave bits (if any) come last.
         dp.ntb = 0;
         std::vector<int> rest;
         for (int t = 0; t < ds.k; ++t)
-          if (reg_index[t] < 0) rest.push_back(t);
+          if (reg_index[t] < 0 && !((wmask >> t) & 1)) rest.push_back(t);
         for (int round = 0; round < 4 && !rest.empty() && !(p.flags & PQS_PLAN_LANES_ASCENDING); ++round) {
           int used = 0;
           for (size_t i = 0; i < rest.size();) {
@@ -498,6 +541,10 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
           }
         }
         for (int t : rest) dp.tb[dp.ntb++] = t;
+        for (int t = 0; t < ds.k; ++t)
+          if ((wmask >> t) & 1) dp.tb[dp.ntb++] = t;
+        if (wmask && dp.ntb != ds.k - kRegBits)
+          return fail(PQS_ERR_UNSUPPORTED, "internal planner error: wave bits overlap the register bits");
       }
       if (p.reg_exec && !ph.lds) {
         // internal consistency: register + thread bits permute the tile bits
