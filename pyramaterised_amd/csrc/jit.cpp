@@ -313,45 +313,74 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       o << "    V a[NA];\n";
       for (int v = 0; v < NA_(); ++v) o << "    a[" << v << "] = tile[pt ^ " << kv[v] << "];\n";
       o << "    double fs = 1.0;\n";
+      // Pending X flips: a CNOT whose control is a thread or constant bit
+      // does not move data; it XORs a per-lane flag xf<r> meaning "logical
+      // register index bit r is stored flipped".  Later ops on r compensate
+      // (X Ry(a) X = Ry(-a), X Rz(a) X = Rz(-a), X Rx X = Rx, CNOT with a
+      // flipped control = CNOT then X on the target, X_a CZ X_a = CZ Z_b),
+      // and the phase's LDS store addresses absorb the flips.
+      bool fl[kRegBits] = {false, false, false, false};
+      o << "    int xf0 = 0, xf1 = 0, xf2 = 0, xf3 = 0;\n";
+      auto xf = [](int r) { return "xf" + std::to_string(r); };
+      auto materialise = [&](int r) {  // apply a pending flip as data movement
+        if (r >= 0 && fl[r]) {
+          o << "    cxc<" << r << ">(a, " << xf(r) << ");\n    " << xf(r) << " = 0;\n";
+          fl[r] = false;
+        }
+      };
       for (int i = P.op_begin; i < P.op_end; ++i) {
         const DevOp &d = p.dev_ops[i];
         const int ai = ang[i - ds.op_begin];
         const bool r0 = d.r0 >= 0, r1 = d.r1 >= 0;
+        // "(xf) ? -t : t" for a rotation whose sign flips with pending X flips
+        auto tsign = [&](const std::string &flip, const std::string &t) {
+          return flip.empty() ? t : "((" + flip + ") ? -" + t + " : " + t + ")";
+        };
         switch (d.kind) {
           case PQS_OP_RY:
+            o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; ry<" << d.r0 << ">(a, "
+              << tsign(fl[d.r0] ? xf(d.r0) : "", "R(c.x)") << "); }\n";
+            break;
           case PQS_OP_RX:
-            o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; " << (d.kind == PQS_OP_RY ? "ry<" : "rx<")
-              << d.r0 << ">(a, R(c.x)); }\n";
+            o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rx<" << d.r0 << ">(a, R(c.x)); }\n";
             break;
           case PQS_OP_RZ:
             if (r0)
-              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rz<" << d.r0 << ">(a, R(c.x)); }\n";
+              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rz<" << d.r0 << ">(a, "
+                << tsign(fl[d.r0] ? xf(d.r0) : "", "R(c.x)") << "); }\n";
             else
               o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; pall(a, " << cbit_expr(d, 0)
                 << " ? R(-c.x) : R(c.x)); }\n";
             break;
-          case PQS_OP_RZZ:
+          case PQS_OP_RZZ: {
+            std::string flip;
+            if (r0 && fl[d.r0]) flip = xf(d.r0);
+            if (r1 && fl[d.r1]) flip = flip.empty() ? xf(d.r1) : "(" + flip + " ^ " + xf(d.r1) + ")";
             if (r0 && r1)
-              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rzz<" << d.r0 << "," << d.r1
-                << ">(a, R(c.x)); }\n";
-            else if (r0 || r1)
-              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rz<" << (r0 ? d.r0 : d.r1) << ">(a, "
-                << cbit_expr(d, r0 ? 1 : 0) << " ? R(-c.x) : R(c.x)); }\n";
-            else
+              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rzz<" << d.r0 << "," << d.r1 << ">(a, "
+                << tsign(flip, "R(c.x)") << "); }\n";
+            else if (r0 || r1) {
+              const std::string cb = cbit_expr(d, r0 ? 1 : 0);
+              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rz<" << (r0 ? d.r0 : d.r1) << ">(a, ("
+                << (flip.empty() ? cb : "(" + cb + " ^ " + flip + ")") << ") ? R(-c.x) : R(c.x)); }\n";
+            } else
               o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; pall(a, (" << cbit_expr(d, 0) << " ^ "
                 << cbit_expr(d, 1) << ") ? R(-c.x) : R(c.x)); }\n";
             break;
+          }
           case PQS_OP_MAT1: {
             auto cv = [&](int e) { return "V{R(" + dbl(d.m[2 * e]) + "), R(" + dbl(d.m[2 * e + 1]) + ")}"; };
-            if (r0)
+            if (r0) {
+              materialise(d.r0);
               o << "    mat1<" << d.r0 << ">(a, " << cv(0) << ", " << cv(1) << ", " << cv(2) << ", " << cv(3)
                 << ");\n";
-            else
+            } else
               o << "    scale(a, " << cbit_expr(d, 0) << " ? " << cv(3) << " : " << cv(0) << ");\n";
             break;
           }
           case kGen1: {
             // c * Pauli on q0, only in the instance carrying this tangent
+            if (r0) materialise(d.r0);
             const double cr = d.m[0], ci = d.m[1];
             double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // m00, m01, m10, m11 (re, im)
             if (d.pauli == 0) { m[0] = cr; m[1] = ci; m[6] = cr; m[7] = ci; }
@@ -367,25 +396,54 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
             break;
           }
           case PQS_OP_CX:
-            if (r0)
+            // q0 = control, q1 = target (a register bit)
+            if (r0) {
               o << "    cxr<" << d.r1 << "," << d.r0 << ">(a);\n";
-            else
-              o << "    cxc<" << d.r1 << ">(a, " << cbit_expr(d, 0) << ");\n";
+              if (fl[d.r0]) {  // flipped control: CNOT then X on the target
+                o << "    " << xf(d.r1) << " ^= " << xf(d.r0) << ";\n";
+                fl[d.r1] = true;
+              }
+            } else {
+              o << "    " << xf(d.r1) << " ^= " << cbit_expr(d, 0) << ";\n";
+              fl[d.r1] = true;
+            }
             break;
           case PQS_OP_CZ:
-            if (r0 && r1)
+            if (r0 && r1) {
               o << "    czr<" << d.r0 << "," << d.r1 << ">(a);\n";
-            else if (r0 || r1)
-              o << "    czc<" << (r0 ? d.r0 : d.r1) << ">(a, " << cbit_expr(d, r0 ? 1 : 0) << ");\n";
-            else
+              // X_a CZ X_a = CZ Z_b; both flipped: -CZ Z_a Z_b
+              if (fl[d.r0]) o << "    czc<" << d.r1 << ">(a, " << xf(d.r0) << ");\n";
+              if (fl[d.r1]) o << "    czc<" << d.r0 << ">(a, " << xf(d.r1) << ");\n";
+              if (fl[d.r0] && fl[d.r1]) o << "    if (" << xf(d.r0) << " & " << xf(d.r1) << ") fs = -fs;\n";
+            } else if (r0 || r1) {
+              const int rr = r0 ? d.r0 : d.r1;
+              const std::string cb = cbit_expr(d, r0 ? 1 : 0);
+              o << "    czc<" << rr << ">(a, " << cb << ");\n";
+              // X_r CZ(r, c) X_r = CZ Z_c: a sign where the constant bit is set
+              if (fl[rr]) o << "    if (" << xf(rr) << " & " << cb << ") fs = -fs;\n";
+            } else {
               o << "    negall(a, " << cbit_expr(d, 0) << " & " << cbit_expr(d, 1) << ");\n";
+            }
             break;
           default:
             break;
         }
       }
+      // stores land where the flipped indices belong
+      {
+        bool any = false;
+        for (int r = 0; r < kRegBits; ++r) any = any || fl[r];
+        if (any) {
+          o << "    const int pst = pt";
+          for (int r = 0; r < kRegBits; ++r)
+            if (fl[r]) o << " ^ (" << xf(r) << " ? " << swz_host(1 << P.rb[r]) << " : 0)";
+          o << ";\n";
+        } else {
+          o << "    const int pst = pt;\n";
+        }
+      }
       o << "    rscale(a, R(fs));\n";
-      for (int v = 0; v < NA_(); ++v) o << "    tile[pt ^ " << kv[v] << "] = a[" << v << "];\n";
+      for (int v = 0; v < NA_(); ++v) o << "    tile[pst ^ " << kv[v] << "] = a[" << v << "];\n";
       // the next phase keeps the wave bits (thread bits 6..): its LDS reads
       // touch only this wave's amplitudes, which this wave's own earlier
       // LDS writes (executed in order) already hold
