@@ -346,6 +346,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-rate", type=float, default=25.0)
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--legs", default="28q,qfim12,serving,speedups",
+                    help="extra legs after the headline (comma list of 28q, qfim12, serving, speedups)")
+    ap.add_argument("--only-legs", action="store_true",
+                    help="skip the headline and the CPU baseline; print the legs' JSON only (for profiling a leg)")
     ap.add_argument("--calib", action="store_true",
                     help="first copy 2 GiB (complex128, 2^27 elements) three times: a plain copy of known size "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -366,24 +370,34 @@ def main():
         log("calibration copy: 3 x (read 2 GiB + write 2 GiB)")
         del a, b
         torch.cuda.empty_cache()
-    log("headline: batched expressibility, 16 qubits, 5000 pairs")
-    head = headline(pyqc, args)
-    log(json.dumps(head))
+    legs = set(x for x in args.legs.split(",") if x)
+    head = None
+    if not args.only_legs:
+        log("headline: batched expressibility, 16 qubits, 5000 pairs")
+        head = headline(pyqc, args)
+        log(json.dumps(head))
     extra = {}
     if not args.quick:
-        for dt in ("complex128", "complex64"):
-            log(f"28-qubit single state, {dt}")
-            extra[f"single_state_28q_{dt}"] = leg_28q(pyqc, dt)
-            log(json.dumps(extra[f"single_state_28q_{dt}"]))
-        log("12-qubit TFIM QFIM")
-        extra["qfim_12q_tfim"] = leg_qfim12(pyqc)
-        log(json.dumps(extra["qfim_12q_tfim"]))
-        log(f"serving mix: {args.serve_requests} requests at {args.serve_rate}/s")
-        extra["serving"] = leg_serving(args)
-        log(json.dumps(extra["serving"]))
-        log("16-qubit speedups vs plain path")
-        extra["speedup_16q"] = speedups_16q(pyqc, args, head)
-        log(json.dumps(extra["speedup_16q"]))
+        if "28q" in legs:
+            for dt in ("complex128", "complex64"):
+                log(f"28-qubit single state, {dt}")
+                extra[f"single_state_28q_{dt}"] = leg_28q(pyqc, dt)
+                log(json.dumps(extra[f"single_state_28q_{dt}"]))
+        if "qfim12" in legs:
+            log("12-qubit TFIM QFIM")
+            extra["qfim_12q_tfim"] = leg_qfim12(pyqc)
+            log(json.dumps(extra["qfim_12q_tfim"]))
+        if "serving" in legs:
+            log(f"serving mix: {args.serve_requests} requests at {args.serve_rate}/s")
+            extra["serving"] = leg_serving(args)
+            log(json.dumps(extra["serving"]))
+        if "speedups" in legs and head is not None:
+            log("16-qubit speedups vs plain path")
+            extra["speedup_16q"] = speedups_16q(pyqc, args, head)
+            log(json.dumps(extra["speedup_16q"]))
+    if args.only_legs:
+        print(json.dumps(extra))
+        return
     log("cpu baseline")
     cpu = cpu_baseline(args)
     log(json.dumps(cpu))

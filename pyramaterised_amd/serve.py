@@ -172,17 +172,21 @@ class Server:
     """Continuous-batching request loop on the current GPU.
 
     hbm_budget: device bytes that admitted, unreleased requests may hold
-    (default: half of the free memory).  n_streams: concurrent HIP streams.
+    (default: half of the device memory; each stream also keeps one Jacobian
+    buffer, sized to the largest QFIM it served, until ``close``).  n_streams: concurrent HIP streams
+    (the first one carries large requests, one at a time; large_bytes is
+    the device footprint from which a QFIM request counts as large).
     max_batch: most requests of one class per launch.  keep_states: keep
     every state-request's state resident (counted in the budget) until
     ``release``; otherwise only Q is returned."""
 
-    def __init__(self, hbm_budget=None, n_streams=4, max_batch=512, dtype="complex128", keep_states=True,
-                 max_inflight_per_stream=2, specialise_qfim_from=14, specialise_state_from=18, compile_workers=8):
+    def __init__(self, hbm_budget=None, n_streams=2, max_batch=512, dtype="complex128", keep_states=True,
+                 max_inflight_per_stream=2, specialise_qfim_from=14, specialise_state_from=18, compile_workers=8,
+                 large_bytes=1 << 30):
         self.dev = E.device()
         if hbm_budget is None:
-            free, _ = torch.cuda.mem_get_info(self.dev)
-            hbm_budget = free // 2
+            _, total = torch.cuda.mem_get_info(self.dev)
+            hbm_budget = total // 2
         self.budget = int(hbm_budget)
         self.in_use = 0
         self.dtype = E.dtype_code(dtype)
@@ -190,6 +194,7 @@ class Server:
         self.max_batch = int(max_batch)
         self.keep_states = keep_states
         self.max_inflight = int(max_inflight_per_stream)
+        self.large_bytes = int(large_bytes)
         self.slots = [_Slot(self.dev) for _ in range(max(1, n_streams))]
         self._next_slot = 0
         self.circuits = {}
@@ -206,6 +211,11 @@ class Server:
         self._pool = ThreadPoolExecutor(max_workers=2)
         self._compiling = set()
         self._seen = {}
+        self._n_pending = 0
+        self._n_inflight = 0
+        self._order = []          # class keys, oldest head-of-queue first
+        self._order_dirty = False
+        self._jscratch = {}       # slot -> reusable Jacobian buffer (stream-ordered reuse)
 
     # ---------------------------------------------------------------- sizing
     def request_bytes(self, req):
@@ -231,12 +241,14 @@ class Server:
                              f"{self.budget}")
         self.results[req.rid] = Result(req)
         self.queues.setdefault((req.n, req.depth, req.kind), deque()).append(req)
+        self._n_pending += 1
+        self._order_dirty = True
 
     def pending(self):
-        return sum(len(q) for q in self.queues.values())
+        return self._n_pending
 
     def inflight(self):
-        return sum(len(s.inflight) for s in self.slots)
+        return self._n_inflight
 
     def release(self, rid):
         """Drop a resident state and return its bytes to the budget."""
@@ -246,22 +258,43 @@ class Server:
         self.in_use -= self._held.pop(rid, 0)
 
     # ---------------------------------------------------------------- loop
-    def _free_slot(self):
-        for k in range(len(self.slots)):
-            s = self.slots[(self._next_slot + k) % len(self.slots)]
+    def _free_slot(self, large):
+        """Large launches (they fill the GPU on their own) share the first
+        stream, one at a time, so they neither compete with each other nor
+        scatter their multi-GiB buffers over several streams' allocator
+        pools; small launches round-robin over the other streams."""
+        if large or len(self.slots) == 1:
+            s = self.slots[0]
+            limit = 1 if large else self.max_inflight
+            return s if len(s.inflight) < limit else None
+        others = self.slots[1:]
+        for k in range(len(others)):
+            s = others[(self._next_slot + k) % len(others)]
             if len(s.inflight) < self.max_inflight:
-                self._next_slot = (self._next_slot + k + 1) % len(self.slots)
+                self._next_slot = (self._next_slot + k + 1) % len(others)
                 return s
         return None
 
+    def is_large(self, req):
+        """A request whose launch alone fills the GPU (a QFIM whose Jacobian
+        exceeds large_bytes)."""
+        return req.kind == QFIM and self.request_bytes(req) >= self.large_bytes
+
     def _admit_and_launch(self, now):
         launched = 0
-        # oldest head-of-queue first
-        keys = sorted((q[0].arrival, k) for k, q in self.queues.items() if q)
-        for _, key in keys:
-            slot = self._free_slot()
+        if self._n_pending == 0:
+            return 0
+        # oldest head-of-queue first (re-sorted only when the queues change)
+        if self._order_dirty:
+            self._order = [k for _, k in sorted((q[0].arrival, k) for k, q in self.queues.items() if q)]
+            self._order_dirty = False
+        for key in self._order:
+            q = self.queues[key]
+            if not q:
+                continue
+            slot = self._free_slot(self.is_large(q[0]))
             if slot is None:
-                break
+                continue
             q = self.queues[key]
             take, need = [], 0
             while q and len(take) < self.max_batch:
@@ -274,6 +307,8 @@ class Server:
                     break  # one Jacobian per launch
             if not take:
                 continue
+            self._n_pending -= len(take)
+            self._order_dirty = True
             self.in_use += need
             self.peak_in_use = max(self.peak_in_use, self.in_use)
             self._launch(slot, key, take, need, now)
@@ -299,7 +334,13 @@ class Server:
                 _, b.q = E.mw(b.out, n)
             else:
                 r = reqs[0]
-                psi, J = E.jacobian(c.best("gplan"), _to_device(r.params, self.dev), c.init)
+                P, dim = 2 * r.n * r.depth, 1 << r.n
+                buf = self._jscratch.get(id(slot))
+                if buf is None or buf.numel() < P * dim:
+                    # grown, never shrunk: launches on one stream reuse it in order
+                    buf = self._jscratch[id(slot)] = torch.empty(P * dim, dtype=E.CDTYPE[self.dtype], device=self.dev)
+                J = buf[:P * dim].view(P, dim)
+                psi, J = E.jacobian(c.best("gplan"), _to_device(r.params, self.dev), c.init, jac_out=J)
                 b.F = E.qfim(J, psi)
                 b.out = psi
                 del J
@@ -308,6 +349,7 @@ class Server:
         for r in reqs:
             self.results[r.rid].launched = now
         slot.inflight.append(b)
+        self._n_inflight += 1
         self.n_batches += 1
         self.batch_sizes.append(len(reqs))
 
@@ -316,6 +358,7 @@ class Server:
         for slot in self.slots:
             while slot.inflight and slot.inflight[0].event.query():
                 b = slot.inflight.popleft()
+                self._n_inflight -= 1
                 kind = b.reqs[0].kind
                 if kind == STATE_MW:
                     qs = b.q.cpu().numpy()
@@ -354,6 +397,7 @@ class Server:
 
     def close(self):
         self._pool.shutdown(wait=True)
+        self._jscratch.clear()
 
     def run(self, requests, realtime=True, clock=time.perf_counter):
         """Serve ``requests`` (sorted by arrival).  realtime: a request
@@ -367,13 +411,16 @@ class Server:
             while i < total and (not realtime or reqs[i].arrival <= now):
                 self.submit(reqs[i])
                 i += 1
-            finished += self._retire(now)
+            done = self._retire(now)
+            finished += done
             self._load_compiled()
-            self._admit_and_launch(now)
+            launched = self._admit_and_launch(now)
             if self.pending() and not self.inflight():
                 # nothing running and nothing fits: released states would
                 # have to make room
                 raise RuntimeError("HBM budget exhausted by resident states; release results")
+            if not done and not launched:
+                time.sleep(50e-6)  # idle: yield the host thread briefly
         self.wall = clock() - t0
         return [self.results[r.rid] for r in requests]
 
