@@ -175,6 +175,26 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *plan, void *states_dev, int64_t state_
                   const int32_t *tangents_host, int32_t init, const void *src_dev,
                   const void *init_table_dev, void *stream);
 
+/* Graph-capturable batched runs.  pqs_batch_table writes the instance table
+ * of a strided batch (arguments as pqs_run_batch; init 0 needs src_dev) into
+ * the caller's device buffer table_dev (batch * pqs_batch_table_bytes(1)
+ * bytes), with alt_dev as the second buffer of every instance when the plan
+ * uses rolling layouts (batch * state_stride amplitudes, else NULL).
+ * pqs_run_table then launches every sweep for that table: no allocation, no
+ * host-to-device copy, nothing but kernel launches on `stream`, so it can
+ * be captured in a HIP graph and replayed; the table records buffer
+ * addresses, so replays see the current contents of states, params and src.
+ * The plan must have been run or tabled once on ctx's device before a
+ * capture (its device image and kernel attributes are set up then).  No
+ * reference counterpart: it serves callers that replay one batch shape many
+ * times (sampling loops, serving). */
+int64_t pqs_batch_table_bytes(int64_t batch);
+int pqs_batch_table(pqs_ctx *ctx, pqs_plan *plan, void *states_dev, int64_t state_stride, int64_t batch,
+                    const double *params_dev, int64_t param_stride, int32_t init, const void *src_dev,
+                    void *alt_dev, void *table_dev, void *stream);
+int pqs_run_table(pqs_ctx *ctx, pqs_plan *plan, const void *table_dev, int64_t batch,
+                  const void *init_table_dev, void *stream);
+
 /* All derivative states in one pass: psi_dev gets U|psi0>, jac_dev (row t =
  * tangent t, n_tangents rows of 2^N) gets every d psi / d theta_t.  Tangent t
  * is spawned from the plain state at the sweep that holds its generator and

@@ -50,6 +50,9 @@ SIGNATURES = {
     "pqs_plan_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     "pqs_plan_sweep": (C.c_int, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     "pqs_plan_sweep_layout": (C.c_int, [vp, i32, C.POINTER(i32), C.POINTER(i32)]),
+    "pqs_batch_table_bytes": (i64, [i64]),
+    "pqs_batch_table": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, i32, vp, vp, vp, vp]),
+    "pqs_run_table": (C.c_int, [vp, vp, vp, i64, vp, vp]),
     "pqs_plan_tangent_sweep": (C.c_int, [vp, i32, C.POINTER(i32)]),
     "pqs_run": (C.c_int, [vp, vp, C.POINTER(Instance), i64, vp, vp]),
     "pqs_run_batch": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, vp, i32, vp, vp, vp]),

@@ -1685,6 +1685,53 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride,
   return PQS_OK;
 }
 
+int64_t pqs_batch_table_bytes(int64_t batch) { return batch < 0 ? 0 : batch * int64_t(sizeof(InstDev)); }
+
+int pqs_batch_table(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride, int64_t batch,
+                    const double *params, int64_t param_stride, int32_t init, const void *src, void *alt,
+                    void *table, void *stream) {
+  if (!ctx || !p || !states || !table) return fail(PQS_ERR_ARG, "NULL ctx, plan, states or table");
+  if (batch <= 0) return fail(PQS_ERR_ARG, "batch must be positive");
+  if (state_stride < (int64_t(1) << p->n)) return fail(PQS_ERR_ARG, "state stride below 2^N");
+  if (init < 0 || init > 2) return fail(PQS_ERR_ARG, "init must be 0, 1 or 2");
+  if (init == 0 && !src) return fail(PQS_ERR_ARG, "init 0 needs a source state for a tabled run");
+  if (p->rolling && !alt) return fail(PQS_ERR_ARG, "rolling-layout plan needs the alt buffer");
+  if (p->n_tangents > 0) return fail(PQS_ERR_UNSUPPORTED, "tabled runs take plans without tangents");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ab = amp_bytes(p->dtype);
+  std::vector<InstDev> h((size_t)batch);
+  for (int64_t i = 0; i < batch; ++i) {
+    InstDev &d = h[(size_t)i];
+    d.state = (char *)states + size_t(i) * size_t(state_stride) * ab;
+    d.src = src;
+    d.params = params ? params + i * param_stride : nullptr;
+    d.alt = p->rolling ? (char *)alt + size_t(i) * size_t(state_stride) * ab : nullptr;
+    d.tangent = -1;
+    d.init = init;
+    d.start_sweep = 0;
+    d.pingpong = p->rolling ? 1 : 0;
+    if (p->rolling && src == d.state) return fail(PQS_ERR_ARG, "tabled rolling run: src must not be the output");
+  }
+  int rc = plan_upload(ctx, p);
+  if (rc) return rc;
+  return upload(ctx, table, h.data(), h.size() * sizeof(InstDev), st);
+}
+
+int pqs_run_table(pqs_ctx *ctx, pqs_plan *p, const void *table, int64_t batch, const void *init_tab,
+                  void *stream) {
+  if (!ctx || !p || !table) return fail(PQS_ERR_ARG, "NULL ctx, plan or table");
+  if (batch <= 0) return fail(PQS_ERR_ARG, "batch must be positive");
+  if (!p->d_ops || p->dev_device != ctx->device) return fail(PQS_ERR_ARG, "plan not on this device: table it first");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
+    int rc = launch_sweep(ctx, p, s, (const InstDev *)table, batch, init_tab, st);
+    if (rc) return rc;
+  }
+  return PQS_OK;
+}
+
 int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, const void *src,
                  const void *init_tab, void *psi, void *jac, void *stream) {
   if (!ctx || !p || !psi) return fail(PQS_ERR_ARG, "NULL ctx, plan or psi");

@@ -409,6 +409,53 @@ def run(plan, params, init, tangents=None, out=None):
     return out
 
 
+class BatchGraph:
+    """All sweep launches of one batched run captured in a HIP graph.
+
+    The instance table is written once into a device buffer owned by this
+    object (``pqs_batch_table``); the capture holds only kernel launches
+    (``pqs_run_table``).  ``replay()`` re-runs the evolution on the same
+    buffers: update ``self.params`` (and the source state, if any) in place
+    between replays.  Rolling-layout plans get their second buffer here."""
+
+    def __init__(self, plan, params, init, out=None):
+        ctx = context()
+        dev = device()
+        self.plan = plan
+        self.params = _params_tensor(params, dev, rows=True).clone()
+        B = self.params.shape[0]
+        dim = 1 << plan.n
+        cdt = CDTYPE[plan.dtype]
+        self.out = out if out is not None else torch.empty((B, dim), dtype=cdt, device=dev)
+        if self.out.shape != (B, dim) or self.out.dtype != cdt or not self.out.is_contiguous():
+            raise ValueError("out has the wrong shape, dtype or layout")
+        code, src, tab = init.device_args(plan.dtype, dev)
+        self._src, self._tab = src, tab
+        self.alt = torch.empty_like(self.out) if plan.rolling else None
+        nb = L.lib().pqs_batch_table_bytes(B)
+        self.table = torch.empty(nb, dtype=torch.uint8, device=dev)
+        check(L.lib().pqs_batch_table(ctx.handle, plan.handle, ptr(self.out), dim, B, ptr(self.params),
+                                      self.params.shape[1], code, ptr(src), ptr(self.alt), ptr(self.table),
+                                      stream_ptr()))
+        self._launch()  # eager once: device image, kernel attributes
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()  # default priority
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(self.graph, stream=side):
+                self._launch()
+        torch.cuda.current_stream().wait_stream(side)
+
+    def _launch(self):
+        check(L.lib().pqs_run_table(context().handle, self.plan.handle, ptr(self.table), self.params.shape[0],
+                                    ptr(self._tab), stream_ptr()))
+
+    def replay(self):
+        self.graph.replay()
+        return self.out
+
+
 def jacobian(plan, params, init, psi_out=None, jac_out=None):
     """psi (2^N,), J (n_tangents, 2^N) in one forward pass."""
     ctx = context()
