@@ -44,6 +44,8 @@ struct InstDev {
 __device__ __forceinline__ V cmul(V a, V b) { return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ __forceinline__ V cadd(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ int swz(int j) { return j ^ (((j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15); }
+// LDS tile element at a byte offset (phase addresses are XORs of byte offsets)
+#define LDSB(o) (*reinterpret_cast<V *>(reinterpret_cast<char *>(tile) + (o)))
 template <int B> __device__ __forceinline__ void ry(V (&a)[NA], R t) {
 #pragma unroll
   for (int v = 0; v < NA; ++v) {
@@ -310,8 +312,9 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
           if (v & (1 << i)) a ^= swz_host(1 << P.rb[i]);
         kv[v] = a;
       }
-      o << "    V a[NA];\n";
-      for (int v = 0; v < NA_(); ++v) o << "    a[" << v << "] = tile[pt ^ " << kv[v] << "];\n";
+      o << "    const int ptb = pt * int(sizeof(V));\n    V a[NA];\n";
+      for (int v = 0; v < NA_(); ++v)
+        o << "    a[" << v << "] = LDSB(ptb ^ (" << kv[v] << " * int(sizeof(V))));\n";
       o << "    double fs = 1.0;\n";
       // Pending X flips: a CNOT whose control is a thread or constant bit
       // does not move data; it XORs a per-lane flag xf<r> meaning "logical
@@ -443,7 +446,9 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         }
       }
       o << "    rscale(a, R(fs));\n";
-      for (int v = 0; v < NA_(); ++v) o << "    tile[pst ^ " << kv[v] << "] = a[" << v << "];\n";
+      o << "    const int pstb = pst * int(sizeof(V));\n";
+      for (int v = 0; v < NA_(); ++v)
+        o << "    LDSB(pstb ^ (" << kv[v] << " * int(sizeof(V)))) = a[" << v << "];\n";
       // the next phase keeps the wave bits (thread bits 6..): its LDS reads
       // touch only this wave's amplitudes, which this wave's own earlier
       // LDS writes (executed in order) already hold
