@@ -105,12 +105,16 @@ template <int BT, int BC> __device__ __forceinline__ void cxr(V (&a)[NA]) {
   }
 }
 template <int BT> __device__ __forceinline__ void cxc(V (&a)[NA], int on) {
+  // per-component selects (a select of whole structs can be lowered
+  // through scratch memory)
 #pragma unroll
   for (int v = 0; v < NA; ++v) {
     if (v & (1 << BT)) continue;
     const V x = a[v], y = a[v | (1 << BT)];
-    a[v] = on ? y : x;
-    a[v | (1 << BT)] = on ? x : y;
+    a[v].x = on ? y.x : x.x;
+    a[v].y = on ? y.y : x.y;
+    a[v | (1 << BT)].x = on ? x.x : y.x;
+    a[v | (1 << BT)].y = on ? x.y : y.y;
   }
 }
 template <int B0, int B1> __device__ __forceinline__ void czr(V (&a)[NA]) {
