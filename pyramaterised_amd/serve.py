@@ -399,10 +399,12 @@ class Server:
         self._pool.shutdown(wait=True)
         self._jscratch.clear()
 
-    def run(self, requests, realtime=True, clock=time.perf_counter):
+    def run(self, requests, realtime=True, clock=time.perf_counter, progress=None, progress_every=1000):
         """Serve ``requests`` (sorted by arrival).  realtime: a request
         becomes visible at its arrival time on the server clock; otherwise
-        all are visible at once.  Returns the results in request order."""
+        all are visible at once.  progress(finished, total, seconds) is
+        called every ``progress_every`` finished requests.  Returns the
+        results in request order."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = clock()
         i, total, finished = 0, len(reqs), 0
@@ -412,6 +414,8 @@ class Server:
                 self.submit(reqs[i])
                 i += 1
             done = self._retire(now)
+            if progress is not None and done and (finished + done) // progress_every > finished // progress_every:
+                progress(finished + done, total, now)
             finished += done
             self._load_compiled()
             launched = self._admit_and_launch(now)
