@@ -121,7 +121,8 @@ class _Circuit:
     def specialise(self, which, pool, workers):
         if which in self._jit or which in self._pending:
             return
-        p = E.Plan(self.qc.n_qubits, self.qc.ops(which == "gplan"), dtype=self.qc.dtype)
+        gen = which == "gplan"
+        p = E.Plan(self.qc.n_qubits, self.qc.ops(gen), dtype=self.qc.dtype, **self.qc.plan_options(gen))
         srcs = p.jit_sweep_sources()
         if not srcs:
             self._jit[which] = None
