@@ -527,6 +527,22 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         std::vector<int> rest;
         for (int t = 0; t < ds.k; ++t)
           if (reg_index[t] < 0 && !((wmask >> t) & 1)) rest.push_back(t);
+        // A CNOT whose control is a lane bit and whose target is a register
+        // bit of the same bank class becomes, in the specialised kernels, a
+        // per-lane store-address flip that folds two lanes of a quarter-wave
+        // onto one bank group.  Among the thread bits of a class, prefer one
+        // that is no such control for lanes 0-3.
+        {
+          std::vector<int> bad(size_t(ds.k), 0);
+          for (int ei : ph.ops) {
+            const ExecOp &e = p.exec[ei];
+            if (e.kind != PQS_OP_CX) continue;
+            const pqs_op &so = ops[e.first];
+            const int tc = tile_index[so.q0], tt = tile_index[so.q1];
+            if (tc >= 0 && tt >= 0 && reg_index[tc] < 0 && reg_index[tt] >= 0) bad[size_t(tc)] |= 1 << (tt & 3);
+          }
+          std::stable_partition(rest.begin(), rest.end(), [&](int t) { return !((bad[size_t(t)] >> (t & 3)) & 1); });
+        }
         for (int round = 0; round < 4 && !rest.empty() && !(p.flags & PQS_PLAN_LANES_ASCENDING); ++round) {
           int used = 0;
           for (size_t i = 0; i < rest.size();) {
