@@ -356,7 +356,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
-              "(the batched workload shards as independent instances per GPU; see DESIGN.md)")
+              "(the batched workloads shard as independent instances per GPU; one GPU is supported)")
         sys.exit(2)
     import torch
     import pyramaterised_amd as pyqc
