@@ -8,6 +8,15 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+def pytest_sessionstart(session):
+    """Build libpqs once if this checkout has none yet (hipcc cross-compiles
+    for gfx950 without a GPU); a built library is used as it is."""
+    from pyramaterised_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from pyramaterised_amd import build
+        build.build()
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
     config.addinivalue_line("markers", "slow: long-running CPU test")
