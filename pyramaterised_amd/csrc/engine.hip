@@ -1105,6 +1105,79 @@ __global__ __launch_bounds__(256) void k_gram_partial(const typename CT<R>::V *_
   if (i < m && j < nn) part[(int64_t(blockIdx.z) * m + i) * nn + j] = double2{accr, acci};
 }
 
+// Real symmetric rank-K update C = R R^T (R: P x K doubles, row stride ld),
+// upper 128 x 128 tiles only, split over K.  256 threads; thread (ty, tx)
+// accumulates rows ty + 16a and columns tx + 16b (a, b < 8) of its tile:
+// per k, 16 LDS reads (8 broadcast, 8 consecutive) feed 64 FMAs.
+constexpr int SYT = 128, SYK = 16;
+
+__global__ __launch_bounds__(256) void k_syrk_partial(const double *__restrict__ R, int64_t ld, int64_t P, int64_t K,
+                                                      int64_t kchunk, const int2 *__restrict__ tiles,
+                                                      double *__restrict__ part) {
+  __shared__ double sa[SYK][SYT + 1];
+  __shared__ double sb[SYK][SYT + 1];
+  const int2 tile = tiles[blockIdx.x];
+  const int64_t i0 = int64_t(tile.x) * SYT, j0 = int64_t(tile.y) * SYT;
+  const int64_t k0 = int64_t(blockIdx.y) * kchunk;
+  const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  // tile loads: thread -> row lr = tid / 2 (0..127), 8 consecutive k at (tid & 1) * 8
+  const int lr = tid >> 1, lk = (tid & 1) * 8;
+  double acc[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+  for (int64_t kk = k0; kk < k1; kk += SYK) {
+    {
+      const int64_t ra = i0 + lr, rb = j0 + lr;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t k = kk + lk + u;
+        sa[lk + u][lr] = (ra < P && k < k1) ? R[ra * ld + k] : 0.0;
+        sb[lk + u][lr] = (rb < P && k < k1) ? R[rb * ld + k] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SYK; ++k) {
+      double x[8], y[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) x[a] = sa[k][ty + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) y[b] = sb[k][tx + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+  double *out = part + int64_t(blockIdx.y) * P * P;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int64_t i = i0 + ty + 16 * a;
+    if (i >= P) continue;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int64_t j = j0 + tx + 16 * b;
+      if (j < P) out[i * P + j] = acc[a][b];
+    }
+  }
+}
+
+// Column-major C[i + j P] = sum over the split of part[z][i][j], for entries
+// of computed (upper) tiles.
+__global__ void k_syrk_reduce(const double *__restrict__ part, int64_t P, int64_t ks, double *__restrict__ C) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= P * P) return;
+  const int64_t i = e / P, j = e - i * P;
+  if (i / SYT > j / SYT) return;
+  double a = 0;
+  for (int64_t z = 0; z < ks; ++z) a += part[z * P * P + e];
+  C[i + j * P] = a;
+}
+
 __global__ void k_gram_reduce(const double2 *__restrict__ part, int64_t mn, int64_t ks, double2 *__restrict__ out) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= mn) return;
@@ -1937,7 +2010,46 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
   if (P <= 0 || dim <= 0) return fail(PQS_ERR_ARG, "bad sizes");
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == PQS_C128 && P >= qfim_syrk_min_p() && 2 * dim <= 0x7fffffff) {
+  if (dtype == PQS_C128 && P < qfim_syrk_min_p()) {
+    // Re(J J^dag) = R R^T, R = J as a real P x 2dim matrix: the upper
+    // tiles of the symmetric product with the register-blocked kernel
+    const int64_t K = 2 * dim;
+    const int64_t nt = (P + SYT - 1) / SYT;
+    std::vector<int2> tl;
+    for (int64_t a = 0; a < nt; ++a)
+      for (int64_t b = a; b < nt; ++b) tl.push_back(int2{int(a), int(b)});
+    const int64_t T = int64_t(tl.size());
+    int64_t ks = (2 * int64_t(ctx->n_cu) + T - 1) / T;
+    ks = std::max<int64_t>(1, std::min<int64_t>(ks, K / 256));
+    int64_t kchunk = (K + ks - 1) / ks;
+    kchunk = (kchunk + SYK - 1) / SYK * SYK;
+    ks = (K + kchunk - 1) / kchunk;
+    const size_t part_b = std::max(gram_part_bytes(ctx, 1, P, dim), size_t(ks * P * P) * sizeof(double));
+    const size_t off_c = (part_b + 255) & ~size_t(255);
+    const size_t off_b = (off_c + size_t(P * P) * sizeof(double) + 255) & ~size_t(255);
+    const size_t off_t = (off_b + size_t(P) * sizeof(double2) + 255) & ~size_t(255);
+    int rc = ws_reserve(ctx, off_t + tl.size() * sizeof(int2));
+    if (rc) return rc;
+    char *ws = (char *)ctx->ws;
+    double *C = (double *)(ws + off_c);
+    double2 *bv = (double2 *)(ws + off_b);
+    int2 *d_tiles = (int2 *)(ws + off_t);
+    rc = upload(ctx, d_tiles, tl.data(), tl.size() * sizeof(int2), st);
+    if (rc) return rc;
+    rc = gram_impl(ctx, dtype, psi, dim, 1, jac, dim, P, dim, bv, (double2 *)ws, st);
+    if (rc) return rc;
+    // the partials reuse the workspace head once the b vector is reduced
+    // (same stream, so in order)
+    hipLaunchKernelGGL(k_syrk_partial, dim3(unsigned(T), unsigned(ks)), dim3(256), 0, st, (const double *)jac, K, P,
+                       K, kchunk, d_tiles, (double *)ws);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_syrk_reduce, dim3(grid_for(P * P, 256)), dim3(256), 0, st, (const double *)ws, P, ks, C);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_qfim_combine_sym, dim3(grid_for(P * P, 256)), dim3(256), 0, st, C, bv, P, F);
+    HIPCHECK(hipGetLastError());
+    return PQS_OK;
+  }
+  if (dtype == PQS_C128 && 2 * dim <= 0x7fffffff) {
     // Re(J J^dag) = R R^T with R = J viewed as a real P x 2dim matrix: one
     // real GEMM (rocBLAS dgemm on the matrix cores), half the flops of the
     // complex Gram.  (dsyrk would halve them again but runs at a sixth of

@@ -325,19 +325,21 @@ def test_take_derivative_and_qfim_from_grad_list(pyqc):
     assert np.max(np.abs(F1 - F2)) < 1e-13
 
 
-def test_qfim_large_p_symmetric_update_path(pyqc):
-    """P >= 768 takes the real-GEMM (rocBLAS dgemm) route of pqs_qfim;
-    against the oracle QFIM and against the split-K Gram kernel route
-    (pqs_gram of J with itself, combined on the host)."""
+@pytest.mark.parametrize("n,p", [(8, 48), (8, 19), (10, 32)])
+def test_qfim_real_gram_paths(pyqc, n, p):
+    """pqs_qfim's real routes: P >= 768 one rocBLAS dgemm (8 x 48 layers:
+    P = 768), smaller P the register-blocked symmetric kernel over upper
+    128-row tiles (P = 304: a ragged last tile; P = 640); against the oracle
+    QFIM and against the complex split-K Gram kernel (pqs_gram of J with
+    itself, combined on the host)."""
     import torch
     from pyramaterised_amd import engine as E
     from pyramaterised_amd import templates as T
-    n, p = 8, 48
     qc = pyqc.PQC(n)
     for l in T.clifford_HE_layers(p, n):
         qc.add_layer(l)
     P = qc.n_true_params
-    assert P == 768
+    assert P == 2 * n * p
     th = np.random.default_rng(21).random(P) * 2 * np.pi
     qc.state = qc.run(list(th))
     psi, J = qc.jacobian()
