@@ -14,9 +14,18 @@
 // Rotations are applied as f * M with f = cos(a/2) and
 // M = [[1, -t], [t, 1]] (R_y), [[1, -i t], [-i t, 1]] (R_x) or
 // diag(1 -/+ i t) (R_z, R_zz), t = tan(a/2).  The scalar f is the same for
-// every amplitude; it is accumulated per phase and applied once.  |cos| is
-// clamped below at 1e-13 (sign kept), which changes an entry of the matrix
-// by at most 1e-13 only when |cos(a/2)| < 1e-13.
+// every amplitude; it is accumulated and applied at the end of each phase.
+// |cos| is clamped below at cmin (sign kept), which changes an entry of the
+// matrix by at most cmin, and only when |cos(a/2)| < cmin.  The scaled
+// matrix has norm 1/|cos|, so amplitudes grow by at most 1/|f| since f was
+// last applied; near a = pi that is up to 1/cmin per rotation.  Every K
+// rotations within a phase the kernel therefore applies f early when |f|
+// has fallen below T.  Between two such checks |f| stays above T * cmin^K,
+// which bounds the growth and keeps f a normal number:
+//   double: cmin 1e-20, K 10, T 1e-100 -> |f| >= 1e-300;
+//   float:  cmin 1e-7,  K 4,  T 1e-9   -> |f| >= 1e-37.
+// With random angles |f| halves per rotation on average, so the early
+// application is rare; the check itself is one compare per K rotations.
 #include <hip/hiprtc.h>
 
 #include <cmath>
@@ -132,6 +141,10 @@ __device__ __forceinline__ void negall(V (&a)[NA], int on) { rscale(a, on ? R(-1
 )PQS";
 
 static constexpr int NA_() { return 1 << kRegBits; }
+// cmin, K and T of the scaled rotation form (comment at the top of this file)
+static double cos_min(const pqs_plan &p) { return p.dtype == PQS_C128 ? 1e-20 : 1e-7; }
+static int rot_per_check(const pqs_plan &p) { return p.dtype == PQS_C128 ? 10 : 4; }
+static double rescale_below(const pqs_plan &p) { return p.dtype == PQS_C128 ? 1e-100 : 1e-9; }
 static int swz_host(int j) { return j ^ (((j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15); }
 
 static std::string dbl(double v) {
@@ -297,7 +310,8 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       o << "    double sn, cs;\n";
       o << "    const int sl = SL" << s << "[i];\n";
       o << "    sincos(0.5 * (OF" << s << "[i] + (sl >= 0 ? SC" << s << "[i] * params[sl] : 0.0)), &sn, &cs);\n";
-      o << "    if (fabs(cs) < 1e-13) cs = cs < 0.0 ? -1e-13 : 1e-13;\n";
+      o << "    if (fabs(cs) < " << dbl(cos_min(p)) << ") cs = cs < 0.0 ? -" << dbl(cos_min(p)) << " : "
+        << dbl(cos_min(p)) << ";\n";
       o << "    tf[i] = double2{sn / cs, cs};\n  }\n";
     }
     o << "  __syncthreads();\n";
@@ -335,10 +349,18 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
           fl[r] = false;
         }
       };
+      int nrot = 0;  // rotations since the last check of the accumulated scale
       for (int i = P.op_begin; i < P.op_end; ++i) {
         const DevOp &d = p.dev_ops[i];
         const int ai = ang[i - ds.op_begin];
         const bool r0 = d.r0 >= 0, r1 = d.r1 >= 0;
+        if (d.kind == PQS_OP_RX || d.kind == PQS_OP_RY || d.kind == PQS_OP_RZ || d.kind == PQS_OP_RZZ) {
+          if (nrot == rot_per_check(p)) {
+            o << "    if (fabs(fs) < " << dbl(rescale_below(p)) << ") { rscale(a, R(fs)); fs = 1.0; }\n";
+            nrot = 0;
+          }
+          ++nrot;
+        }
         // "(xf) ? -t : t" for a rotation whose sign flips with pending X flips
         auto tsign = [&](const std::string &flip, const std::string &t) {
           return flip.empty() ? t : "((" + flip + ") ? -" + t + " : " + t + ")";

@@ -120,6 +120,35 @@ def test_jit_product_init_and_edge_angles(pyqc):
 
 
 @pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 1e-5)])
+def test_jit_many_rotations_at_pi_in_one_phase(pyqc, dtype, tol):
+    """Deep stacks of rotations at and next to pi, where each unscaled
+    rotation grows amplitudes by up to 1/cos(a/2): without intermediate
+    scaling, c64 overflows after three of them in one register phase and
+    c128 after about two dozen."""
+    rng = np.random.default_rng(13)
+    n = 12
+    spec = []
+    for _ in range(10):  # no entanglers: all layers fuse into the same phases
+        spec += [("R_y", q) for q in range(n)]
+    for layer in range(8):
+        spec += [(("R_x", "R_y", "R_z")[layer % 3], q) for q in range(n)]
+        spec += [("CZ", (q, q + 1)) for q in range(layer % 2, n - 1, 2)]
+        spec += [("R_zz", (q, q + 1)) for q in range((layer + 1) % 2, n - 1, 2)]
+    pk = [getattr(pyqc, nm)(list(q) if isinstance(q, tuple) else q, n) for nm, q in spec]
+    og = [getattr(OC, nm)(q, n) for nm, q in spec]
+    qc = _build(pyqc, n, [pk], True, dtype=dtype)
+    assert qc.plan().jitted
+    P = qc.n_true_params
+    for th in (np.full(P, np.pi), np.pi + rng.choice([-1e-9, 0.0, 1e-9], size=P),
+               rng.choice([np.pi, -np.pi, 3 * np.pi], size=P)):
+        st = qc.run(list(th)).full().ravel()
+        assert np.all(np.isfinite(st))
+        ref = OS.run(og, th, n)
+        assert infidelity(st, ref) < tol
+        assert np.max(np.abs(st - ref)) < (1e-12 if dtype == "complex128" else 1e-5)
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 1e-5)])
 def test_jit_tile_shapes_persistent_and_tuner(pyqc, dtype, tol):
     """Every tile shape the tuner times, the persistent form, and the tuned
     default, against the oracle on a batch."""
