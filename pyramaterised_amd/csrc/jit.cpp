@@ -14,7 +14,8 @@
 // Rotations are applied as f * M with f = cos(a/2) and
 // M = [[1, -t], [t, 1]] (R_y), [[1, -i t], [-i t, 1]] (R_x) or
 // diag(1 -/+ i t) (R_z, R_zz), t = tan(a/2).  The scalar f is the same for
-// every amplitude; it is accumulated and applied at the end of each phase.
+// every amplitude; it is accumulated over the sweep's phases and applied
+// once at the end (or at the end of a phase that flips its sign per lane).
 // |cos| is clamped below at cmin (sign kept), which changes an entry of the
 // matrix by at most cmin, and only when |cos(a/2)| < cmin.  The scaled
 // matrix has norm 1/|cos|, so amplitudes grow by at most 1/|f| since f was
@@ -315,9 +316,16 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       o << "    tf[i] = double2{sn / cs, cs};\n  }\n";
     }
     o << "  __syncthreads();\n";
+    // fs: the accumulated scale of the rotations (uniform over the block
+    // unless a phase flips its sign per lane); it carries across phases and
+    // is applied at the end of the sweep, at the end of a phase with
+    // per-lane sign flips, or early by the growth check
+    o << "  double fs = 1.0;\n";
+    int nrot = 0;  // rotations since the last check of the accumulated scale
     // phases
     for (int ph = ds.phase_begin; ph < ds.phase_end; ++ph) {
       const DevPhase &P = p.dev_phases[ph];
+      bool lane_sign = false;  // this phase flips fs per lane
       o << "  {\n";
       o << "    int ptid = tid;\n    asm volatile(\"\" : \"+v\"(ptid));\n";
       o << "    const int toff = 0";
@@ -333,7 +341,6 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       o << "    const int ptb = pt * int(sizeof(V));\n    V a[NA];\n";
       for (int v = 0; v < NA_(); ++v)
         o << "    a[" << v << "] = LDSB(ptb ^ (" << kv[v] << " * int(sizeof(V))));\n";
-      o << "    double fs = 1.0;\n";
       // Pending X flips: a CNOT whose control is a thread or constant bit
       // does not move data; it XORs a per-lane flag xf<r> meaning "logical
       // register index bit r is stored flipped".  Later ops on r compensate
@@ -349,7 +356,6 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
           fl[r] = false;
         }
       };
-      int nrot = 0;  // rotations since the last check of the accumulated scale
       for (int i = P.op_begin; i < P.op_end; ++i) {
         const DevOp &d = p.dev_ops[i];
         const int ai = ang[i - ds.op_begin];
@@ -443,13 +449,19 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
               // X_a CZ X_a = CZ Z_b; both flipped: -CZ Z_a Z_b
               if (fl[d.r0]) o << "    czc<" << d.r1 << ">(a, " << xf(d.r0) << ");\n";
               if (fl[d.r1]) o << "    czc<" << d.r0 << ">(a, " << xf(d.r1) << ");\n";
-              if (fl[d.r0] && fl[d.r1]) o << "    if (" << xf(d.r0) << " & " << xf(d.r1) << ") fs = -fs;\n";
+              if (fl[d.r0] && fl[d.r1]) {
+                o << "    if (" << xf(d.r0) << " & " << xf(d.r1) << ") fs = -fs;\n";
+                lane_sign = true;
+              }
             } else if (r0 || r1) {
               const int rr = r0 ? d.r0 : d.r1;
               const std::string cb = cbit_expr(d, r0 ? 1 : 0);
               o << "    czc<" << rr << ">(a, " << cb << ");\n";
               // X_r CZ(r, c) X_r = CZ Z_c: a sign where the constant bit is set
-              if (fl[rr]) o << "    if (" << xf(rr) << " & " << cb << ") fs = -fs;\n";
+              if (fl[rr]) {
+                o << "    if (" << xf(rr) << " & " << cb << ") fs = -fs;\n";
+                lane_sign = true;
+              }
             } else {
               o << "    negall(a, " << cbit_expr(d, 0) << " & " << cbit_expr(d, 1) << ");\n";
             }
@@ -471,7 +483,10 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
           o << "    const int pst = pt;\n";
         }
       }
-      o << "    rscale(a, R(fs));\n";
+      if (lane_sign || ph + 1 == ds.phase_end) {
+        o << "    rscale(a, R(fs));\n    fs = 1.0;\n";
+        nrot = 0;
+      }
       o << "    const int pstb = pst * int(sizeof(V));\n";
       for (int v = 0; v < NA_(); ++v)
         o << "    LDSB(pstb ^ (" << kv[v] << " * int(sizeof(V)))) = a[" << v << "];\n";
