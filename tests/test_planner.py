@@ -74,6 +74,36 @@ def test_library_loads_and_exports_every_header_symbol():
     assert set(L.SIGNATURES) == names
 
 
+def test_entry_points_reject_null_context_without_touching_the_gpu():
+    """Every compute entry point checks its context / plan / buffer
+    arguments first: a NULL context returns PQS_ERR_ARG (< 0) with a
+    message, and nothing is launched (this runs without a GPU)."""
+    import ctypes as C
+    lib = L.lib()
+    skip = {"pqs_abi_version", "pqs_last_error", "pqs_ctx_create", "pqs_ctx_destroy", "pqs_plan_destroy",
+            "pqs_plan_create", "pqs_batch_table_bytes", "pqs_jit_prepare"}
+    checked = 0
+    for name, (res, args) in L.SIGNATURES.items():
+        if name in skip:
+            continue
+        vals = []
+        for t in args:
+            if t in (C.c_int, C.c_int32, C.c_int64):
+                vals.append(0)
+            elif t is C.c_double:
+                vals.append(0.0)
+            else:
+                vals.append(None)
+        rc = getattr(lib, name)(*vals)
+        assert rc < 0, name
+        assert lib.pqs_last_error(), name
+        checked += 1
+    assert checked >= 20
+    # destroying NULL is a no-op, as free(NULL) is
+    assert lib.pqs_ctx_destroy(None) == 0
+    assert lib.pqs_plan_destroy(None) == 0
+
+
 def test_plan_rejects_bad_ops(pyqc):
     from pyramaterised_amd import engine as E
     from pyramaterised_amd.gates import _op
