@@ -543,7 +543,21 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
           }
           std::stable_partition(rest.begin(), rest.end(), [&](int t) { return !((bad[size_t(t)] >> (t & 3)) & 1); });
         }
-        for (int round = 0; round < 4 && !rest.empty() && !(p.flags & PQS_PLAN_LANES_ASCENDING); ++round) {
+        // Lane bits 0-2 address one group of 8 lanes: a 16-byte LDS read is
+        // conflict-free when the group's slots differ mod 16 (their bank
+        // classes differ), a write when they differ mod 8 (class 3 is
+        // bit 3 of the slot, which writes do not separate).  So the first
+        // round takes classes 0, 1, 2 and then 3.
+        if (!(p.flags & PQS_PLAN_LANES_ASCENDING)) {
+          for (int cls = 0; cls < 4; ++cls)
+            for (size_t i = 0; i < rest.size(); ++i)
+              if ((rest[i] & 3) == cls) {
+                dp.tb[dp.ntb++] = rest[i];
+                rest.erase(rest.begin() + long(i));
+                break;
+              }
+        }
+        for (int round = 1; round < 4 && !rest.empty() && !(p.flags & PQS_PLAN_LANES_ASCENDING); ++round) {
           int used = 0;
           for (size_t i = 0; i < rest.size();) {
             const int cls = rest[i] & 3;
