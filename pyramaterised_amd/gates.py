@@ -227,6 +227,11 @@ class Gate:
         ops = self._const_ops(th, ph)
         return LocalOperator(self.q_N, ops)
 
+    def get_op(self):
+        """The gate's operator at its current angles (gates.py:122 and the
+        other get_op methods); ``operation`` is the same, built on access."""
+        return self.operation
+
     def _const_ops(self, th, ph):
         out = []
         for o in self.prim_ops(0):
@@ -663,8 +668,35 @@ def _fsim(t, f):
 
 
 def fsim_gate(theta, phi, N=None, control=0, target=1):
-    """gates.py:588-606 (local 4x4)."""
+    """gates.py:588-606 (local 4x4; the engine places it on its qubits)."""
     return _fsim(theta, phi)
+
+
+def fsim_gate_d_theta(theta, phi, N=None, control=0, target=1):
+    """gates.py:609-627, as written: the middle block differentiated in
+    theta, the corners of fsim_gate kept (local 4x4)."""
+    m = _fsim(theta, phi)
+    m[1, 1] = m[2, 2] = -np.sin(theta)
+    m[1, 2] = m[2, 1] = -1j * np.cos(theta)
+    return m
+
+
+def fsim_gate_d_phi(theta, phi, N=None, control=0, target=1):
+    """gates.py:630-648, as written: fsim_gate with its last entry
+    differentiated in phi (local 4x4)."""
+    m = _fsim(theta, phi)
+    m[3, 3] = -1j * np.exp(-1j * phi)
+    return m
+
+
+def fixed_fsim_gate(theta, N=None, control=0, target=1):
+    """gates.py:700-716: fsim_gate at phi = 0 (local 4x4)."""
+    return _fsim(theta, 0.0)
+
+
+def fixed_fsim_gate_d_theta(theta, N=None, control=0, target=1):
+    """gates.py:719-737, as written (local 4x4)."""
+    return fsim_gate_d_theta(theta, 0.0)
 
 
 class fSim(PRot):
@@ -701,12 +733,7 @@ class fSim(PRot):
         o.offset, o.scale, o.slot = float(self.theta), 0.0, -1
         # phi from a constant: encode as a dense generator-free matrix
         th, ph = float(self.theta), float(self.phi)
-        m = _fsim(th, ph)
-        if which == 1:
-            m[1, 1] = m[2, 2] = -np.sin(th)
-            m[1, 2] = m[2, 1] = -1j * np.cos(th)
-        else:
-            m[3, 3] = -1j * np.exp(-1j * ph)
+        m = fsim_gate_d_theta(th, ph) if which == 1 else fsim_gate_d_phi(th, ph)
         return [_op(L.OP_MAT2, self.q1, self.q2, m=m)]
 
     def derivative(self):
@@ -738,10 +765,7 @@ class fixed_fSim(PRot):
         return [_op(L.OP_GEN_FSIM, self.q1, self.q2, slot=slot, slot2=-1, tangent=tangent, pauli=3)]
 
     def derivative(self):
-        th = float(self.theta)
-        m = _fsim(th, 0.0)
-        m[1, 1] = m[2, 2] = -np.sin(th)
-        m[1, 2] = m[2, 1] = -1j * np.cos(th)
+        m = fixed_fsim_gate_d_theta(float(self.theta))
         return LocalOperator(self.q_N, [_op(L.OP_MAT2, self.q1, self.q2, m=m)])
 
     def flip_pauli(self):

@@ -206,10 +206,44 @@ class Measurements:
         state = self.QC.run(angles=angles)
         return -1 * self.gkp_fast(state)
 
+    # The reference evaluates the stabiliser Renyi sum as an O(4^n) dot
+    # product over two precomputed 2^n x 2^n tables (measure.py:268-313).  The
+    # engine's kernel (magic.py) needs neither; the table builders are kept
+    # for callers that build, pass or inspect them.
+    def numberToBase(self, n, b, n_qubits):
+        """measure.py:268-277: the n_qubits base-b digits of n, most
+        significant first; IndexError when n needs more digits."""
+        digits = np.zeros(n_qubits, dtype=int)
+        k = n_qubits
+        while n:
+            k -= 1
+            if k < 0:
+                raise IndexError(f"{n} has more than {n_qubits} digits in base {b}")
+            digits[k] = int(n % b)
+            n //= b
+        return digits
+
+    def get_conversion_matrix_mod_add_index(self, base_states):
+        """measure.py:279-291: entry [j, k] is the index (most significant
+        digit first) of base state k plus base state j, digit-wise mod 2."""
+        B = np.asarray(base_states, dtype=int)
+        weights = 2 ** np.arange(B.shape[1])[::-1]
+        out = np.zeros((len(B), len(B)), dtype=int)
+        for j in range(len(B)):
+            out[j] = ((B + B[j]) % 2) @ weights
+        return out
+
+    def get_conversion_matrix_binary_prod(self, base_states):
+        """measure.py:293-302: entry [i, k] is (-1)^(b_i . b_k mod 2)."""
+        B = np.asarray(base_states, dtype=int)
+        return 1 - 2 * ((B @ B.T) % 2)
+
     def get_conversion_matrices(self):
-        """The reference precomputes O(4^n) index/sign tables
-        (measure.py:268-313); the GPU kernel needs none."""
-        return None
+        """measure.py:304-313: (mod-add index table, sign table) for this
+        circuit's n, each 2^n x 2^n in host memory."""
+        n = self.QC.n_qubits
+        base = [self.numberToBase(i, 2, n) for i in range(2 ** n)]
+        return (self.get_conversion_matrix_mod_add_index(base), self.get_conversion_matrix_binary_prod(base))
 
     def set_converstion_matrices(self, conv_mats):
         self.conversion_matrices = conv_mats

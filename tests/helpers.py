@@ -78,3 +78,19 @@ def infidelity(a, b):
 
 def rel_fro(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def sre_bruteforce(psi, n, alpha):
+    """Stabiliser Renyi entropy of order alpha by explicit Pauli strings:
+    1/(1-alpha) log sum_P (<P>^2 / d)^alpha - log d."""
+    import itertools
+    P = [np.eye(2), np.array([[0, 1], [1, 0]]), np.array([[0, -1j], [1j, 0]]), np.diag([1, -1])]
+    d = 2 ** n
+    s = 0.0
+    for codes in itertools.product(range(4), repeat=n):
+        m = np.array([[1.0 + 0j]])
+        for c in codes:
+            m = np.kron(m, P[c])
+        e = np.vdot(psi, m @ psi).real
+        s += (e * e / d) ** alpha
+    return 1 / (1 - alpha) * np.log(s) - np.log(d)

@@ -1,0 +1,61 @@
+"""Host-side compatibility helpers that need no GPU: the reference's fSim
+matrix helpers and its magic conversion tables."""
+import numpy as np
+
+from tests.helpers import sre_bruteforce
+
+
+def test_fsim_helper_matrices():
+    """gates.py:588-737.  The "derivative" matrices are the reference's as
+    written: the differentiated block, with the other entries of fsim_gate
+    kept.  Pinned by central differences of fsim_gate and by unitarity."""
+    import pyramaterised_amd as pyqc
+    h = 1e-6
+    for th, ph in ((0.3, 1.1), (2.5, -0.7), (np.pi / 2, np.pi)):
+        U = pyqc.fsim_gate(th, ph)
+        assert np.allclose(U.conj().T @ U, np.eye(4), atol=1e-14)
+        dt = pyqc.fsim_gate_d_theta(th, ph)
+        fd = (pyqc.fsim_gate(th + h, ph) - pyqc.fsim_gate(th - h, ph)) / (2 * h)
+        assert np.max(np.abs(dt[1:3, 1:3] - fd[1:3, 1:3])) < 1e-8
+        keep = np.ones((4, 4), bool)
+        keep[1:3, 1:3] = False
+        assert np.array_equal(dt[keep], U[keep])
+        dp = pyqc.fsim_gate_d_phi(th, ph)
+        fd = (pyqc.fsim_gate(th, ph + h) - pyqc.fsim_gate(th, ph - h)) / (2 * h)
+        assert abs(dp[3, 3] - fd[3, 3]) < 1e-8
+        keep = np.ones((4, 4), bool)
+        keep[3, 3] = False
+        assert np.array_equal(dp[keep], U[keep])
+        assert np.array_equal(pyqc.fixed_fsim_gate(th), pyqc.fsim_gate(th, 0.0))
+        assert np.array_equal(pyqc.fixed_fsim_gate_d_theta(th), pyqc.fsim_gate_d_theta(th, 0.0))
+
+
+def test_magic_conversion_tables_and_reference_formula():
+    """measure.py:268-349: the index / sign tables, checked entry by entry
+    against XOR and popcount parity, and the reference's O(4^n) Renyi
+    formula over them against the sum over explicit Pauli strings."""
+    import pyramaterised_amd as pyqc
+    n = 3
+    m = pyqc.measure.Measurements(pyqc.PQC(n))
+    assert list(m.numberToBase(5, 2, n)) == [1, 0, 1]
+    assert list(m.numberToBase(0, 2, n)) == [0, 0, 0]
+    assert list(m.numberToBase(7, 3, 3)) == [0, 2, 1]
+    try:
+        m.numberToBase(8, 2, n)
+        raise AssertionError("expected IndexError")
+    except IndexError:
+        pass
+    add, sign = m.get_conversion_matrices()
+    d = 2 ** n
+    assert add.shape == sign.shape == (d, d)
+    for i in range(d):
+        for k in range(d):
+            assert add[i, k] == i ^ k
+            assert sign[i, k] == (-1) ** bin(i & k).count("1")
+    rng = np.random.default_rng(3)
+    psi = rng.normal(size=d) + 1j * rng.normal(size=d)
+    psi /= np.linalg.norm(psi)
+    for alpha in (2, 0.5):
+        s = np.sum(np.abs(2 ** (-n / 2) * np.dot(np.conjugate(psi) * sign, psi[add])) ** (2 * alpha))
+        val = 1 / (1 - alpha) * np.log(s) - np.log(d)
+        assert abs(val - sre_bruteforce(psi, n, alpha)) < 1e-12

@@ -7,7 +7,7 @@ import pytest
 from oracle import circuits as OC
 from oracle import measures as OM
 from oracle import sim as OS
-from tests.helpers import build_oracle, build_pkg, infidelity, random_spec, rel_fro
+from tests.helpers import build_oracle, build_pkg, infidelity, random_spec, rel_fro, sre_bruteforce
 
 pytestmark = pytest.mark.gpu
 
@@ -273,29 +273,14 @@ def test_expect_pauli_and_tfim_hamiltonian(pyqc):
     assert abs(pyqc.expect(Hy, ket) - np.vdot(v, Hy.full() @ v).real) < 1e-13
 
 
-def _sre_bruteforce(psi, n, alpha):
-    """sum over Pauli strings of (<P>^2/d)^alpha by explicit strings."""
-    import itertools
-    P = [np.eye(2), np.array([[0, 1], [1, 0]]), np.array([[0, -1j], [1j, 0]]), np.diag([1, -1])]
-    d = 2 ** n
-    s = 0.0
-    for codes in itertools.product(range(4), repeat=n):
-        m = np.array([[1.0 + 0j]])
-        for c in codes:
-            m = np.kron(m, P[c])
-        e = np.vdot(psi, m @ psi).real
-        s += (e * e / d) ** alpha
-    return 1 / (1 - alpha) * np.log(s) - np.log(d)
-
-
 def test_magic_matches_pauli_bruteforce(pyqc):
     rng = np.random.default_rng(8)
     n = 4
     v = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
     v /= np.linalg.norm(v)
     m = pyqc.measure.Measurements(pyqc.PQC(n))
-    assert abs(m.renyi_entropy_fast(v) - _sre_bruteforce(v, n, 2)) < 1e-12
-    assert abs(m.gkp_fast(v) - _sre_bruteforce(v, n, 0.5) / (2 * np.log(2))) < 1e-12
+    assert abs(m.renyi_entropy_fast(v) - sre_bruteforce(v, n, 2)) < 1e-12
+    assert abs(m.gkp_fast(v) - sre_bruteforce(v, n, 0.5) / (2 * np.log(2))) < 1e-12
     # Haar average: log(3 + 2^n) - log 4 (tests.py:276-277), statistically
     states = rng.normal(size=(400, 2 ** n)) + 1j * rng.normal(size=(400, 2 ** n))
     states /= np.linalg.norm(states, axis=1, keepdims=True)
