@@ -101,6 +101,11 @@ enum {
                                    with a workgroup barrier (default: phases that
                                    keep the same wave bits exchange through LDS
                                    within each wave, without one) */
+  PQS_PLAN_NO_DIRECT = 32,      /* specialised kernels: stage every tile through
+                                   LDS at both ends of a sweep (default: the first
+                                   / last register phase loads / stores its
+                                   registers directly when the tile bits at
+                                   positions 0 and 1 are lane bits there) */
   PQS_PLAN_ROLLING = 8          /* rolling layouts: each sweep writes the state
                                    with the qubit -> bit-position map under which
                                    the next sweep's tile is one contiguous block;

@@ -277,24 +277,43 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     o << "      if (" << s << " == in.start_sweep) { src = in.src ? (const V *)in.src : dst; init = in.src ? 0 : in.init; }\n";
     o << "      else src = dst;\n";
     o << "    }\n  };\n";
+    const bool persist = (p.flags & PQS_PLAN_JIT_PERSISTENT) != 0;
+    // Direct load / store: the first (last) phase's registers come from (go
+    // to) HBM without staging the tile in LDS; its lane bits 0-3 are the
+    // tile bits at positions 0-3, so each 16 lanes touch 16 contiguous
+    // amplitudes.  Otherwise the tile is staged with thread bit i = tile bit
+    // i and register bit b = tile bit kt + b.
+    const bool dload = ds.direct_load && !persist && ds.phase_end > ds.phase_begin;
+    const bool dstore = ds.direct_store && !persist && ds.phase_end > ds.phase_begin;
+    int lthr[kMaxTileBits], lreg[kRegBits];  // global positions of the load's thread / register bits
+    for (int i = 0; i < kt; ++i) lthr[i] = tpos[dload ? p.dev_phases[ds.phase_begin].tb[i] : i];
+    for (int b = 0; b < kRegBits; ++b) lreg[b] = tpos[dload ? p.dev_phases[ds.phase_begin].rb[b] : kt + b];
+    auto lhp = [&](int m) {
+      unsigned long long g = 0;
+      for (int b = 0; b < kRegBits; ++b)
+        if (m & (1 << b)) g |= 1ull << lreg[b];
+      return g;
+    };
+    o << "  const u64 ltid = 0ull";
+    for (int i = 0; i < kt; ++i) o << " | (u64((tid >> " << i << ") & 1) << " << lthr[i] << ")";
+    o << ";\n";
     o << "  auto fetch = [&](long long ft, V (&pre)[NA]) {\n";
     o << "    const InstDev in = insts[ft >> " << (p.n - k) << "];\n";
     o << "    const V *src; V *dst; int init;\n    io(in, src, dst, init);\n";
-    o << "    const u64 dl = tile_base(ft) | dtid;\n";
+    o << "    const u64 dl = tile_base(ft) | ltid;\n";
     o << "    if (init == 0) {\n";
-    for (int m = 0; m < NA_(); ++m) o << "      pre[" << m << "] = src[dl | " << hp(m) << "ull];\n";
+    for (int m = 0; m < NA_(); ++m) o << "      pre[" << m << "] = src[dl | " << lhp(m) << "ull];\n";
     o << "    } else if (init == 1) {\n";
     for (int m = 0; m < NA_(); ++m)
-      o << "      pre[" << m << "] = V{(dl | " << hp(m) << "ull) == 0ull ? R(1) : R(0), R(0)};\n";
+      o << "      pre[" << m << "] = V{(dl | " << lhp(m) << "ull) == 0ull ? R(1) : R(0), R(0)};\n";
     o << "    } else {\n";
     o << "#pragma unroll\n      for (int m = 0; m < NA; ++m) {\n";
     o << "        u64 gi = dl;\n";
-    for (int b = 0; b < kRegBits; ++b) o << "        if (m & " << (1 << b) << ") gi |= " << (1ull << tpos[kt + b]) << "ull;\n";
+    for (int b = 0; b < kRegBits; ++b) o << "        if (m & " << (1 << b) << ") gi |= " << (1ull << lreg[b]) << "ull;\n";
     o << "        V v{R(1), R(0)};\n";
     o << "        for (int q = 0; q < " << p.n << "; ++q) v = cmul(v, init_tab[2 * q + int((gi >> (" << (p.n - 1)
       << " - q)) & 1ull)]);\n";
     o << "        pre[m] = v;\n      }\n    }\n  };\n";
-    const bool persist = (p.flags & PQS_PLAN_JIT_PERSISTENT) != 0;
     o << "  V pre[NA];\n  fetch(t, pre);\n";
     if (persist) o << "  for (; t < total; t += gridDim.x) {\n";
     else o << "  {\n";
@@ -304,7 +323,8 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     o << "  const u64 dl = base | dtid;\n";
     o << "  const double *params = in.params;\n";
     o << "  __syncthreads();\n";
-    o << "#pragma unroll\n  for (int m = 0; m < NA; ++m) tile[swz(tid + m * " << nthr << ")] = pre[m];\n";
+    if (!dload)
+      o << "#pragma unroll\n  for (int m = 0; m < NA; ++m) tile[swz(tid + m * " << nthr << ")] = pre[m];\n";
     if (persist) o << "  if (t + gridDim.x < total) fetch(t + gridDim.x, pre);\n";
     if (na > 0) {
       o << "  for (int i = tid; i < " << na << "; i += " << nthr << ") {\n";
@@ -339,8 +359,12 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         kv[v] = a;
       }
       o << "    const int ptb = pt * int(sizeof(V));\n    V a[NA];\n";
-      for (int v = 0; v < NA_(); ++v)
-        o << "    a[" << v << "] = LDSB(ptb ^ (" << kv[v] << " * int(sizeof(V))));\n";
+      if (dload && ph == ds.phase_begin) {
+        for (int v = 0; v < NA_(); ++v) o << "    a[" << v << "] = pre[" << v << "];\n";
+      } else {
+        for (int v = 0; v < NA_(); ++v)
+          o << "    a[" << v << "] = LDSB(ptb ^ (" << kv[v] << " * int(sizeof(V))));\n";
+      }
       // Pending X flips: a CNOT whose control is a thread or constant bit
       // does not move data; it XORs a per-lane flag xf<r> meaning "logical
       // register index bit r is stored flipped".  Later ops on r compensate
@@ -487,6 +511,33 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         o << "    rscale(a, R(fs));\n    fs = 1.0;\n";
         nrot = 0;
       }
+      if (dstore && ph + 1 == ds.phase_end) {
+        // direct store: tile bit j goes to write position wp[j]; flipped
+        // register bits flip their position bit per lane
+        int wp[kMaxTileBits];
+        for (int j = 0; j < k; ++j) wp[j] = ds.rolling ? ds.wpos[j] : tpos[j];
+        if (ds.rolling) {
+          o << "    const u64 wtid = u64(t) & " << (ds.n_tiles - 1) << "ull;\n";
+          o << "    const u64 gb = 0ull";
+          for (int j = 0; j < p.n - k; ++j) o << " | (((wtid >> " << j << ") & 1ull) << " << ds.cwpos[j] << ")";
+          o << ";\n";
+        } else {
+          o << "    const u64 gb = tile_base(t);\n";
+        }
+        o << "    const u64 gt = gb";
+        for (int i = 0; i < P.ntb; ++i) o << " | (u64((ptid >> " << i << ") & 1) << " << wp[P.tb[i]] << ")";
+        for (int r = 0; r < kRegBits; ++r)
+          if (fl[r]) o << " ^ (" << xf(r) << " ? " << (1ull << wp[P.rb[r]]) << "ull : 0ull)";
+        o << ";\n";
+        for (int v = 0; v < NA_(); ++v) {
+          unsigned long long cm = 0;
+          for (int b = 0; b < kRegBits; ++b)
+            if (v & (1 << b)) cm |= 1ull << wp[P.rb[b]];
+          o << "    dst[gt ^ " << cm << "ull] = a[" << v << "];\n";
+        }
+        o << "  }\n";
+        continue;
+      }
       o << "    const int pstb = pst * int(sizeof(V));\n";
       for (int v = 0; v < NA_(); ++v)
         o << "    LDSB(pstb ^ (" << kv[v] << " * int(sizeof(V)))) = a[" << v << "];\n";
@@ -506,7 +557,9 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       else
         o << "    __syncthreads();\n  }\n";
     }
-    if (!ds.rolling) {
+    if (dstore) {
+      // stored by the last phase
+    } else if (!ds.rolling) {
       for (int m = 0; m < NA_(); ++m) o << "  dst[dl | " << hp(m) << "ull] = tile[swz(tid + " << m * nthr << ")];\n";
     } else {
       // write layout: store index bit i is tile bit sperm[i], at write

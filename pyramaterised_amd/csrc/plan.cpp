@@ -461,6 +461,16 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     // specialised kernels exchange through LDS without a workgroup barrier
     // inside the run.  Per run, the wave bits are the tile bits needed
     // latest.
+    // tile bit read / written at positions 0-3 (-1 if none); the first /
+    // last register phase wants them as lane bits
+    int rbit[4] = {-1, -1, -1, -1}, wbit[4] = {-1, -1, -1, -1};
+    for (int t = 0; t < ds.k; ++t) {
+      if (tpos[size_t(t)] < 4) rbit[tpos[size_t(t)]] = t;
+      const int w = ds.rolling ? ds.wpos[t] : tpos[size_t(t)];
+      if (w < 4) wbit[w] = t;
+    }
+    // direct loads / stores (specialised kernels; not with persistent blocks)
+    const bool direct_ok = !(p.flags & (PQS_PLAN_JIT_PERSISTENT | PQS_PLAN_NO_DIRECT));
     const int nwb = std::max(0, ds.k - kRegBits - 6);
     std::vector<int> phase_w(phases.size(), 0);  // tile-bit mask of the wave bits
     if (p.reg_exec && nwb > 0 && !(p.flags & PQS_PLAN_NO_WAVE_LOCAL)) {
@@ -488,15 +498,41 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         std::stable_sort(cand.begin(), cand.end(), [](const std::pair<size_t, int> &a, const std::pair<size_t, int> &b) {
           return a.first > b.first;
         });
-        int W = 0;
-        for (int c = 0; c < nwb && c < int(cand.size()); ++c) W |= 1 << cand[size_t(c)].second;
-        size_t j = i;
-        while (j < phases.size() && !phases[j].lds && !(rt[j] & W)) phase_w[j++] = W;
+        // the tile bits at positions 0 and 1 should stay lane bits in the
+        // first phase (read layout) and the last (write layout), where the
+        // specialised kernels load / store registers directly
+        auto pick = [&](int avoid) {
+          std::vector<std::pair<size_t, int>> c2 = cand;
+          std::stable_partition(c2.begin(), c2.end(), [&](const std::pair<size_t, int> &x) { return !((avoid >> x.second) & 1); });
+          int W = 0;
+          for (int c = 0; c < nwb && c < int(c2.size()); ++c) W |= 1 << c2[size_t(c)].second;
+          return W;
+        };
+        auto extent = [&](int W) {
+          size_t j = i;
+          while (j < phases.size() && !phases[j].lds && !(rt[j] & W)) ++j;
+          return j;
+        };
+        auto bits01 = [](const int *b) { return (b[0] >= 0 ? 1 << b[0] : 0) | (b[1] >= 0 ? 1 << b[1] : 0); };
+        const int avoid = (i == 0 && direct_ok) ? bits01(rbit) : 0;
+        int W = pick(avoid);
+        size_t j = extent(W);
+        if (j == phases.size() && direct_ok) {
+          const int W2 = pick(avoid | bits01(wbit));
+          const size_t j2 = extent(W2);
+          if (j2 > i) {
+            W = W2;
+            j = j2;
+          }
+        }
+        for (size_t q = i; q < j; ++q) phase_w[q] = W;
+        if (j == i) ++j;  // (W excludes phase i's register bits, so phase i always fits)
         i = j;
       }
     }
     size_t phase_no = 0;
     for (const PhaseSpec &ph : phases) {
+      const bool first = phase_no == 0, last = phase_no + 1 == phases.size();
       const int wmask = phase_w[phase_no++];
       DevPhase dp{};
       dp.op_begin = int(p.dev_ops.size());
@@ -543,18 +579,36 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
           }
           std::stable_partition(rest.begin(), rest.end(), [&](int t) { return !((bad[size_t(t)] >> (t & 3)) & 1); });
         }
+        // First / last register phase: the tile bits at read / write
+        // positions 0-3 that are thread bits here become the first lane bits
+        // (in bank-class order), provided positions 0 and 1 are among them:
+        // then every 4 lanes of a direct load / store touch 64 contiguous
+        // bytes.
+        auto in_rest = [&](int t) { return t >= 0 && std::find(rest.begin(), rest.end(), t) != rest.end(); };
+        const int *lowb = (first && !ph.lds) ? rbit : (last && !ph.lds) ? wbit : nullptr;
+        if (lowb && in_rest(lowb[0]) && in_rest(lowb[1]) && direct_ok) {
+          int want = 0;
+          for (int i = 0; i < 4; ++i)
+            if (in_rest(lowb[i])) want |= 1 << lowb[i];
+          for (int cls = 0; cls < 4; ++cls)
+            for (int t = 0; t < ds.k; ++t)
+              if (((want >> t) & 1) && (t & 3) == cls) dp.tb[dp.ntb++] = t;
+          rest.erase(std::remove_if(rest.begin(), rest.end(), [&](int t) { return (want >> t) & 1; }), rest.end());
+        }
         // Lane bits 0-2 address one group of 8 lanes: a 16-byte LDS read is
         // conflict-free when the group's slots differ mod 16 (their bank
         // classes differ), a write when they differ mod 8 (class 3 is
         // bit 3 of the slot, which writes do not separate).  So the first
         // round takes classes 0, 1, 2 and then 3.
-        if (!(p.flags & PQS_PLAN_LANES_ASCENDING)) {
+        if (!(p.flags & PQS_PLAN_LANES_ASCENDING) && dp.ntb < 4) {
+          int used = 0;
+          for (int i = 0; i < dp.ntb; ++i) used |= 1 << (dp.tb[i] & 3);
           for (int cls = 0; cls < 4; ++cls)
-            for (size_t i = 0; i < rest.size(); ++i)
+            for (size_t i = 0; i < rest.size() && !((used >> cls) & 1); ++i)
               if ((rest[i] & 3) == cls) {
                 dp.tb[dp.ntb++] = rest[i];
+                used |= 1 << cls;
                 rest.erase(rest.begin() + long(i));
-                break;
               }
         }
         for (int round = 1; round < 4 && !rest.empty() && !(p.flags & PQS_PLAN_LANES_ASCENDING); ++round) {
@@ -573,6 +627,15 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         for (int t : rest) dp.tb[dp.ntb++] = t;
         for (int t = 0; t < ds.k; ++t)
           if ((wmask >> t) & 1) dp.tb[dp.ntb++] = t;
+        if (direct_ok) {
+          auto is_lane = [&](int t) {
+            for (int i = 0; i < std::min(6, dp.ntb); ++i)
+              if (dp.tb[i] == t) return true;
+            return false;
+          };
+          if (first && rbit[0] >= 0 && rbit[1] >= 0 && is_lane(rbit[0]) && is_lane(rbit[1])) ds.direct_load = 1;
+          if (last && wbit[0] >= 0 && wbit[1] >= 0 && is_lane(wbit[0]) && is_lane(wbit[1])) ds.direct_store = 1;
+        }
         if (wmask && dp.ntb != ds.k - kRegBits)
           return fail(PQS_ERR_UNSUPPORTED, "internal planner error: wave bits overlap the register bits");
       }

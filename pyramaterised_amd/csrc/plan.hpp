@@ -89,6 +89,10 @@ struct DevSweep {
   int32_t n;  // qubits of the state
   int32_t sperm[kMaxTileBits], wpos[kMaxTileBits];
   int32_t cwpos[kMaxQubits];
+  // Lane bits 0-3 of the first (last) register phase are the tile bits read
+  // (written) at positions 0-3, so the specialised kernel loads (stores)
+  // that phase's registers directly, without staging the tile in LDS.
+  int32_t direct_load, direct_store;
 };
 
 std::string plan_error(); // last planner error (thread local)

@@ -200,6 +200,43 @@ def test_rolling_layouts_match_oracle(pyqc, dtype, tol, jit):
     assert np.max(np.abs(one - st[1])) < (1e-13 if dtype == "complex128" else 1e-6)
 
 
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 1e-5)])
+def test_direct_load_store_match_staged_tiles(pyqc, dtype, tol):
+    """Sweeps whose first / last register phase loads / stores registers
+    directly (no LDS staging) against the same plans staged through LDS
+    (PQS_PLAN_NO_DIRECT) and the oracle: rolling and fixed layouts, |0>,
+    product and vector inputs, a batch."""
+    from pyramaterised_amd import _lib as L
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    n, p, S = 18, 8, 3
+    og = OC.ry_cz_brickwork(p, n)
+    params = np.random.default_rng(42).random((S, n * p)) * 2 * np.pi
+    plus = OS.product_state(np.array([1, 1]) / np.sqrt(2), n)
+    for base in (L.PQS_PLAN_ROLLING, 0):
+        outs = {}
+        for fl in (base, base | L.PQS_PLAN_NO_DIRECT):
+            qc = _build(pyqc, n, T.ry_cz_brickwork_layers(p, n), True, dtype=dtype, tile_bits=12, low_bits=4, flags=fl)
+            plan = qc.plan(batch=S)
+            assert plan.jitted
+            srcs = [x.decode() for x in plan.jit_sweep_sources()]
+            direct = sum("= pre[0];" in x for x in srcs), sum("dst[gt ^" in x for x in srcs)
+            if fl & L.PQS_PLAN_NO_DIRECT:
+                assert direct == (0, 0)
+            else:  # (this fixed-layout plan has no sweep that can store directly)
+                assert direct[0] > 0 and (direct[1] > 0 or not base), direct
+            st = qc.run_batch(params).cpu().numpy()
+            qc.set_initial_state(T.plus_state())
+            sp = qc.run(list(params[0])).full().ravel()
+            outs[fl] = (st, sp)
+            for s in range(S):
+                assert infidelity(st[s], OS.run(og, params[s], n)) < tol
+            assert infidelity(sp, OS.run(og, params[0], n, plus)) < tol
+        a, b = outs.values()
+        lim = 1e-13 if dtype == "complex128" else 1e-6
+        assert np.max(np.abs(a[0] - b[0])) < lim and np.max(np.abs(a[1] - b[1])) < lim
+
+
 def test_rolling_all_reg_kinds_and_inplace_input(pyqc):
     """All register op kinds under rolling layouts; a vector input state;
     and the C ABI's in-place form (init 0, no source: the state buffer is
