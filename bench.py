@@ -340,10 +340,17 @@ def cpu_baseline(args):
     for i in range(0, k - 1, 2):
         abs(np.vdot(sts[i], sts[i + 1])) ** 2
     dt = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": k * 940 / dt, "unit": "gates/s", "cores": 1, "kind": "port",
             "sample": f"{k} of the 10,000 states of the headline workload (16 qubits, 940 gates each) "
                       f"and their pair fidelity: per-gate CSR operators rebuilt per parameter + spmv, "
-                      f"scipy single thread, {dt:.1f} s"}
+                      f"scipy single thread, {dt:.1f} s",
+            "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
 def main():
@@ -354,7 +361,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=5000)
     ap.add_argument("--plain-states", type=int, default=200)
     ap.add_argument("--plain-overlaps", type=int, default=2000)
-    ap.add_argument("--cpu-states", type=int, default=4)
+    ap.add_argument("--cpu-states", type=int, default=12)
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-rate", type=float, default=25.0)
     ap.add_argument("--quick", action="store_true", help="headline only")
