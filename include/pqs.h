@@ -234,9 +234,9 @@ int pqs_gram(pqs_ctx *ctx, int32_t dtype, const void *a_dev, int64_t lda, int64_
 
 /* QFIM from a Jacobian and the state: F_pq = 4 Re(<J_p|J_q> - <J_p|psi><psi|J_q>)
  * into f_dev (P x P float64).  Replaces Measurements.get_QFI
- * (measure.py:33-71).  complex128 with P >= 768 (PQS_QFIM_SYRK_MIN_P) forms
- * Re(J J^dag) as one real GEMM of J viewed as a P x 2^(N+1) real matrix
- * (rocBLAS dgemm); smaller P use the split-K Gram kernel. */
+ * (measure.py:33-71).  complex128 forms Re(J J^dag) = R R^T, R = J viewed as a
+ * P x 2^(N+1) real matrix, over the upper 128 x 128 tiles on the FP64 matrix
+ * cores, split over K; complex64 uses the complex split-K Gram kernel. */
 int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac_dev, int64_t P,
              const void *psi_dev, int64_t dim, double *f_dev, void *stream);
 

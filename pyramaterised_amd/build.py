@@ -34,7 +34,7 @@ def build(verbose=False):
            "-Wno-unused-command-line-argument",
            "-I", os.path.join(ROOT, "include"), "-I", CSRC]
     cmd += [os.path.join(CSRC, s) for s in SOURCES]
-    cmd += ["-L/opt/rocm/lib", "-lhiprtc", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp]
+    cmd += ["-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

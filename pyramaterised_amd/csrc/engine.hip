@@ -16,7 +16,6 @@
 // two-stage with a fixed order so that every result is bit-identical whatever
 // else runs in the same batch.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <sys/stat.h>
 
@@ -1106,64 +1105,83 @@ __global__ __launch_bounds__(256) void k_gram_partial(const typename CT<R>::V *_
 }
 
 // Real symmetric rank-K update C = R R^T (R: P x K doubles, row stride ld),
-// upper 128 x 128 tiles only, split over K.  256 threads; thread (ty, tx)
-// accumulates rows ty + 16a and columns tx + 16b (a, b < 8) of its tile:
-// per k, 16 LDS reads (8 broadcast, 8 consecutive) feed 64 FMAs.
-constexpr int SYT = 128, SYK = 16;
+// upper 128 x 128 tiles only, split over K, on the FP64 matrix cores.
+constexpr int SYT = 128;
 
-__global__ __launch_bounds__(256) void k_syrk_partial(const double *__restrict__ R, int64_t ld, int64_t P, int64_t K,
-                                                      int64_t kchunk, const int2 *__restrict__ tiles,
-                                                      double *__restrict__ part) {
-  __shared__ double sa[SYK][SYT + 1];
-  __shared__ double sb[SYK][SYT + 1];
+// v_mfma_f64_16x16x4f64 takes,
+// in lane l, A[l % 16][l / 16] and B[l / 16][l % 16] and returns
+// D[4 r + l / 16][l % 16] in element r.  256 threads: wave w owns the 64 x 64
+// quadrant (w / 2, w % 2) of the 128 x 128 tile as 4 x 4 blocks of 16 x 16.
+// K advances in steps of 16: the rows' k-slices are staged row-major in LDS
+// (row stride 18 doubles: the 32 lanes of one operand read hit 64 distinct
+// banks) and the next step's slices are loaded into registers while the
+// current step runs.  Accumulators 128 registers, occupancy 2 waves / SIMD.
+typedef double d4_t __attribute__((ext_vector_type(4)));
+constexpr int MFK = 16, MFLD = MFK + 2;
+
+__global__ __launch_bounds__(256, 2) void k_syrk_mfma(const double *__restrict__ R, int64_t ld, int64_t P, int64_t K,
+                                                   int64_t kchunk, const int2 *__restrict__ tiles,
+                                                   double *__restrict__ part) {
+  __shared__ double sa[SYT * MFLD];
+  __shared__ double sb[SYT * MFLD];
   const int2 tile = tiles[blockIdx.x];
   const int64_t i0 = int64_t(tile.x) * SYT, j0 = int64_t(tile.y) * SYT;
   const int64_t k0 = int64_t(blockIdx.y) * kchunk;
   const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  // tile loads: thread -> row lr = tid / 2 (0..127), 8 consecutive k at (tid & 1) * 8
-  const int lr = tid >> 1, lk = (tid & 1) * 8;
-  double acc[8][8];
+  const int t = threadIdx.x, l = t & 63, w = t >> 6;
+  const int qr = (w >> 1) * 64, qc = (w & 1) * 64;
+  d4_t acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
-  for (int64_t kk = k0; kk < k1; kk += SYK) {
-    {
-      const int64_t ra = i0 + lr, rb = j0 + lr;
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
+  // staging: element e = t + 256 u is row e / 8, doubles 2 (e % 8) + {0, 1}
+  // of the step's 16 (K, kchunk and the steps are even, so a pair is
+  // either inside [k0, k1) or outside it)
+  double2 pa[4], pb[4];
+  auto gload = [&](int64_t kk) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t k = kk + lk + u;
-        sa[lk + u][lr] = (ra < P && k < k1) ? R[ra * ld + k] : 0.0;
-        sb[lk + u][lr] = (rb < P && k < k1) ? R[rb * ld + k] : 0.0;
-      }
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u, row = e >> 3, c2 = e & 7;
+      const int64_t k = kk + 2 * c2, ra = i0 + row, rb = j0 + row;
+      pa[u] = (ra < P && k < k1) ? *reinterpret_cast<const double2 *>(R + ra * ld + k) : double2{0.0, 0.0};
+      pb[u] = (rb < P && k < k1) ? *reinterpret_cast<const double2 *>(R + rb * ld + k) : double2{0.0, 0.0};
+    }
+  };
+  if (k0 < k1) gload(k0);
+  for (int64_t kk = k0; kk < k1; kk += MFK) {
+    __syncthreads();  // the previous step's operand reads are done
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u, row = e >> 3, c2 = e & 7;
+      *reinterpret_cast<double2 *>(&sa[row * MFLD + 2 * c2]) = pa[u];
+      *reinterpret_cast<double2 *>(&sb[row * MFLD + 2 * c2]) = pb[u];
     }
     __syncthreads();
+    if (kk + MFK < k1) gload(kk + MFK);
 #pragma unroll
-    for (int k = 0; k < SYK; ++k) {
-      double x[8], y[8];
+    for (int k4 = 0; k4 < MFK; k4 += 4) {
+      double af[4], bf[4];
 #pragma unroll
-      for (int a = 0; a < 8; ++a) x[a] = sa[k][ty + 16 * a];
+      for (int a = 0; a < 4; ++a) af[a] = sa[(qr + 16 * a + (l & 15)) * MFLD + k4 + (l >> 4)];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) y[b] = sb[k][tx + 16 * b];
+      for (int b = 0; b < 4; ++b) bf[b] = sb[(qc + 16 * b + (l & 15)) * MFLD + k4 + (l >> 4)];
 #pragma unroll
-      for (int a = 0; a < 8; ++a)
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
-    __syncthreads();
   }
   double *out = part + int64_t(blockIdx.y) * P * P;
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int64_t i = i0 + ty + 16 * a;
-    if (i >= P) continue;
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const int64_t j = j0 + tx + 16 * b;
-      if (j < P) out[i * P + j] = acc[a][b];
-    }
-  }
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + qr + 16 * a + 4 * r + (l >> 4), j = j0 + qc + 16 * b + (l & 15);
+        if (i < P && j < P) out[i * P + j] = acc[a][b][r];
+      }
 }
 
 // Column-major C[i + j P] = sum over the split of part[z][i][j], for entries
@@ -1353,7 +1371,6 @@ struct pqs_ctx {
   size_t pin_bytes = 0;
   hipEvent_t pin_event = nullptr;
   bool pin_pending = false;
-  rocblas_handle blas = nullptr;  // created on the first large QFIM
 };
 
 namespace {
@@ -1619,7 +1636,6 @@ int pqs_ctx_destroy(pqs_ctx *ctx) {
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->pin_event) (void)hipEventDestroy(ctx->pin_event);
-  if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
   delete ctx;
   return PQS_OK;
 }
@@ -1979,16 +1995,6 @@ int gram_impl(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m
   return PQS_OK;
 }
 
-// Smallest P for which pqs_qfim forms Re(J J^dag) with rocBLAS dgemm
-// (PQS_QFIM_SYRK_MIN_P; below it the split-K Gram kernel is faster).
-int64_t qfim_syrk_min_p() {
-  static const int64_t v = [] {
-    const char *e = std::getenv("PQS_QFIM_SYRK_MIN_P");
-    return e ? std::atoll(e) : int64_t(768);
-  }();
-  return v;
-}
-
 }  // namespace
 
 extern "C" {
@@ -2010,9 +2016,11 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
   if (P <= 0 || dim <= 0) return fail(PQS_ERR_ARG, "bad sizes");
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == PQS_C128 && P < qfim_syrk_min_p()) {
+  if (dtype == PQS_C128) {
     // Re(J J^dag) = R R^T, R = J as a real P x 2dim matrix: the upper
-    // tiles of the symmetric product with the register-blocked kernel
+    // tiles of the symmetric product on the matrix cores, split over K
+    // (rocBLAS dgemm on these tall-skinny shapes ran 2-10x slower, erratic
+    // in P, and computes both triangles)
     const int64_t K = 2 * dim;
     const int64_t nt = (P + SYT - 1) / SYT;
     std::vector<int2> tl;
@@ -2022,7 +2030,7 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
     int64_t ks = (2 * int64_t(ctx->n_cu) + T - 1) / T;
     ks = std::max<int64_t>(1, std::min<int64_t>(ks, K / 256));
     int64_t kchunk = (K + ks - 1) / ks;
-    kchunk = (kchunk + SYK - 1) / SYK * SYK;
+    kchunk = (kchunk + MFK - 1) / MFK * MFK;
     ks = (K + kchunk - 1) / kchunk;
     const size_t part_b = std::max(gram_part_bytes(ctx, 1, P, dim), size_t(ks * P * P) * sizeof(double));
     const size_t off_c = (part_b + 255) & ~size_t(255);
@@ -2040,40 +2048,11 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
     if (rc) return rc;
     // the partials reuse the workspace head once the b vector is reduced
     // (same stream, so in order)
-    hipLaunchKernelGGL(k_syrk_partial, dim3(unsigned(T), unsigned(ks)), dim3(256), 0, st, (const double *)jac, K, P,
-                       K, kchunk, d_tiles, (double *)ws);
+    hipLaunchKernelGGL(k_syrk_mfma, dim3(unsigned(T), unsigned(ks)), dim3(256), 0, st, (const double *)jac, K, P, K,
+                       kchunk, d_tiles, (double *)ws);
     HIPCHECK(hipGetLastError());
     hipLaunchKernelGGL(k_syrk_reduce, dim3(grid_for(P * P, 256)), dim3(256), 0, st, (const double *)ws, P, ks, C);
     HIPCHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_qfim_combine_sym, dim3(grid_for(P * P, 256)), dim3(256), 0, st, C, bv, P, F);
-    HIPCHECK(hipGetLastError());
-    return PQS_OK;
-  }
-  if (dtype == PQS_C128 && 2 * dim <= 0x7fffffff) {
-    // Re(J J^dag) = R R^T with R = J viewed as a real P x 2dim matrix: one
-    // real GEMM (rocBLAS dgemm on the matrix cores), half the flops of the
-    // complex Gram.  (dsyrk would halve them again but runs at a sixth of
-    // dgemm's rate for these tall-skinny shapes on gfx950.)
-    const size_t part_b = gram_part_bytes(ctx, 1, P, dim);
-    const size_t off_c = (part_b + 255) & ~size_t(255);
-    const size_t off_b = (off_c + size_t(P * P) * sizeof(double) + 255) & ~size_t(255);
-    int rc = ws_reserve(ctx, off_b + size_t(P) * sizeof(double2));
-    if (rc) return rc;
-    char *ws = (char *)ctx->ws;
-    double *C = (double *)(ws + off_c);
-    double2 *bv = (double2 *)(ws + off_b);
-    rc = gram_impl(ctx, dtype, psi, dim, 1, jac, dim, P, dim, bv, (double2 *)ws, st);
-    if (rc) return rc;
-    if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
-      ctx->blas = nullptr;
-      return fail(PQS_ERR_HIP, "rocblas_create_handle failed");
-    }
-    if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success) return fail(PQS_ERR_HIP, "rocblas_set_stream failed");
-    const double one = 1.0, zero = 0.0;
-    if (rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, rocblas_int(P),
-                      rocblas_int(P), rocblas_int(2 * dim), &one, (const double *)jac, rocblas_int(2 * dim),
-                      (const double *)jac, rocblas_int(2 * dim), &zero, C, rocblas_int(P)) != rocblas_status_success)
-      return fail(PQS_ERR_HIP, "rocblas_dgemm failed");
     hipLaunchKernelGGL(k_qfim_combine_sym, dim3(grid_for(P * P, 256)), dim3(256), 0, st, C, bv, P, F);
     HIPCHECK(hipGetLastError());
     return PQS_OK;

@@ -312,11 +312,10 @@ def test_take_derivative_and_qfim_from_grad_list(pyqc):
 
 @pytest.mark.parametrize("n,p", [(8, 48), (8, 19), (10, 32)])
 def test_qfim_real_gram_paths(pyqc, n, p):
-    """pqs_qfim's real routes: P >= 768 one rocBLAS dgemm (8 x 48 layers:
-    P = 768), smaller P the register-blocked symmetric kernel over upper
-    128-row tiles (P = 304: a ragged last tile; P = 640); against the oracle
-    QFIM and against the complex split-K Gram kernel (pqs_gram of J with
-    itself, combined on the host)."""
+    """pqs_qfim's real route: Re(J J^dag) over upper 128-row tiles on the
+    matrix cores, split over K (P = 768: six full tiles; P = 304: a ragged
+    last tile; P = 640); against the oracle QFIM and against the complex
+    split-K Gram kernel (pqs_gram of J with itself, combined on the host)."""
     import torch
     from pyramaterised_amd import engine as E
     from pyramaterised_amd import templates as T
