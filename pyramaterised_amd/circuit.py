@@ -186,13 +186,24 @@ class PQC:
         self._plans[key] = p
         return p
 
+    # Jacobian plans use rolling layouts (fewer sweeps) when the ping-pong
+    # rows they need beside the Jacobian stay within this many bytes
+    ROLLING_JACOBIAN_BYTES = 4 << 30
+
+    def rolling_jacobian(self, dtype=None):
+        """Whether derivative plans use rolling layouts: each Jacobian row
+        then needs a workspace row of the same size."""
+        dt = self.dtype if dtype is None else E.dtype_code(dtype)
+        amp = 16 if dt == E.L.PQS_C128 else 8
+        return self.n_true_params * (amp << self.n_qubits) <= self.ROLLING_JACOBIAN_BYTES
+
     def plan_options(self, with_generators=False, dtype=None):
         """Plan options for an untuned plan: the constructor's, or for
-        derivative plans (whose Jacobian rows stay in place, so no rolling
-        layouts) 12-bit c128 / 13-bit c64 tiles with 128 B runs, which keep
-        two workgroups per CU and leave the planner room (measured at 20
-        qubits, P = 1600: 332 ms against 427 ms for 13-bit tiles with 5
-        fixed low qubits)."""
+        derivative plans 12-bit c128 / 13-bit c64 tiles with 128 B runs,
+        which keep two workgroups per CU and leave the planner room
+        (measured at 20 qubits, P = 1600: 332 ms against 427 ms for 13-bit
+        tiles with 5 fixed low qubits), with rolling layouts when their
+        ping-pong rows fit (rolling_jacobian)."""
         if any(self.plan_opts.values()) or not with_generators:
             return dict(self.plan_opts)
         dt = self.dtype if dtype is None else E.dtype_code(dtype)
@@ -201,6 +212,8 @@ class PQC:
             opts.update(tile_bits=min(12, self.n_qubits), low_bits=3)
         else:
             opts.update(tile_bits=min(13, self.n_qubits), low_bits=4)
+        if self.rolling_jacobian(dt):
+            opts.update(flags=opts.get("flags", 0) | E.L.PQS_PLAN_ROLLING)
         return opts
 
     def _want_tune(self, with_generators):

@@ -1828,7 +1828,6 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
   if (init < 0 || init > 2) return fail(PQS_ERR_ARG, "init must be 0, 1 or 2");
   if (init == 0 && !src) return fail(PQS_ERR_ARG, "init 0 needs a source state");
   if (init == 2 && !init_tab) return fail(PQS_ERR_ARG, "product init needs an init table");
-  if (p->rolling) return fail(PQS_ERR_UNSUPPORTED, "rolling-layout plan in pqs_jacobian");
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
   const int S = int(p->dev_sweeps.size());
@@ -1836,15 +1835,22 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
   const size_t ab = amp_bytes(p->dtype);
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
-  // workspace: [alt state | instance table]
-  const size_t alt_b = (S > 1) ? size_t(dim) * ab : 0;
-  const size_t alt_off = 0;
-  const size_t tab_off = (alt_b + 255) & ~size_t(255);
+  // Rolling layouts write every sweep out of place, so each tangent row
+  // ping-pongs with a workspace row as the primal does; fixed layouts update
+  // the rows in place.  Workspace: [alt state | alt rows | instance table]
+  const bool rows_pp = p->rolling && S > 1;
+  if (rows_pp && init == 0 && src == psi)
+    return fail(PQS_ERR_ARG, "with a rolling-layout plan the source state must not be the psi buffer");
+  const size_t sb = size_t(dim) * ab;
+  const size_t alt_b = (S > 1) ? sb : 0;
+  const size_t rows_off = (alt_b + 255) & ~size_t(255);
+  const size_t rows_b = rows_pp ? size_t(p->n_tangents) * sb : 0;
+  const size_t tab_off = (rows_off + rows_b + 255) & ~size_t(255);
   const size_t tb = size_t(1 + p->n_tangents) * sizeof(InstDev);
   rc = ws_reserve(ctx, tab_off + tb);
   if (rc) return rc;
   char *ws = (char *)ctx->ws;
-  void *alt = S > 1 ? (void *)(ws + alt_off) : nullptr;
+  void *alt = S > 1 ? (void *)ws : nullptr;
   auto W = [&](int s) -> void * { return ((S - 1 - s) & 1) == 0 ? psi : alt; };
 
   std::vector<int> order;
@@ -1870,7 +1876,8 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
     d.params = params;
     d.tangent = t;
     d.start_sweep = s0;
-    d.pingpong = 0;
+    d.pingpong = rows_pp ? 1 : 0;
+    d.alt = rows_pp ? (void *)(ws + rows_off + size_t(t) * sb) : nullptr;
     if (s0 == 0) {
       d.src = init == 0 ? src : nullptr;
       d.init = init;

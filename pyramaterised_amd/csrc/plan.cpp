@@ -189,7 +189,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
   // `low` of them) become the lowest bit positions of the layout in between,
   // so both the contiguous tile reads and the write runs are preserved while
   // the tile moves freely over the qubits.
-  p.rolling = (o.flags & PQS_PLAN_ROLLING) && n > k && k >= kRegBits + 6 && p.n_tangents == 0;
+  p.rolling = (o.flags & PQS_PLAN_ROLLING) && n > k && k >= kRegBits + 6;
 
   // One sweep: greedy over the remaining ops with `want` in the tile first;
   // if `want` leaves no room for the next op, without it.

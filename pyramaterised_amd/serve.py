@@ -226,7 +226,10 @@ class Server:
         if req.kind == STATE_MW:
             return self.amp * dim + 64 * req.n
         P = 2 * req.n * req.depth
-        return self.amp * dim * (P + 2) + 48 * P * P
+        rows = P + 2
+        if P * self.amp * dim <= PQC.ROLLING_JACOBIAN_BYTES:
+            rows += P  # ping-pong rows of a rolling-layout Jacobian (workspace)
+        return self.amp * dim * rows + 48 * P * P
 
     def circuit(self, n, depth):
         key = (n, depth)

@@ -201,6 +201,43 @@ def test_rolling_layouts_match_oracle(pyqc, dtype, tol, jit):
 
 
 @pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 1e-5)])
+def test_rolling_jacobian_matches_fixed_and_oracle(pyqc, dtype, tol):
+    """Forward-mode Jacobian under rolling layouts (every tangent row
+    ping-pongs with a workspace row) against the fixed-layout plan and the
+    oracle's per-parameter re-simulation, from |0> and from |+>^n; and the
+    QFIM from it."""
+    from pyramaterised_amd import _lib as L
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    n, p = 14, 8
+    og = OC.clifford_he(p, n)
+    th = np.random.default_rng(43).random(2 * n * p) * 2 * np.pi
+    plus = OS.product_state(np.array([1, 1]) / np.sqrt(2), n)
+    lim = 1e-12 if dtype == "complex128" else 2e-5
+    for init in (None, plus):
+        dref = OS.derivative_states(og, th, n, init)
+        ref = OS.run(og, th, n, init)
+        got = {}
+        for fl in (L.PQS_PLAN_ROLLING, 0):
+            qc = _build(pyqc, n, T.clifford_HE_layers(p, n), True, dtype=dtype, tile_bits=10, low_bits=3, flags=fl)
+            if init is not None:
+                qc.set_initial_state(T.plus_state())
+            qc.state = qc.run(list(th))
+            psi, J = qc.jacobian()
+            plan = qc.plan(with_generators=True)
+            assert plan.rolling == bool(fl) and plan.jitted
+            got[fl] = (psi.cpu().numpy(), J.cpu().numpy())
+            assert infidelity(got[fl][0], ref) < tol
+            for t in range(0, len(dref), 5):
+                assert np.max(np.abs(got[fl][1][t] - dref[t])) < lim, (fl, t)
+            if dtype == "complex128":
+                F = pyqc.measure.Measurements(qc).get_QFI()
+                assert rel_fro(F, OM.qfim(dref, ref)) < 1e-9
+        a, b = got.values()
+        assert np.max(np.abs(a[1] - b[1])) < lim
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 1e-5)])
 def test_direct_load_store_match_staged_tiles(pyqc, dtype, tol):
     """Sweeps whose first / last register phase loads / stores registers
     directly (no LDS staging) against the same plans staged through LDS

@@ -58,6 +58,8 @@ def active_qubits(o):
         return {o.q1}
     if k in (L.OP_RX, L.OP_RY):
         return {o.q0}
+    if k == L.OP_GEN:  # generator term: Pauli on q0 (and q1)
+        return set() if o.pauli in (0, 3) else {q for q in (o.q0, o.q1) if q >= 0}
     return {o.q0, o.q1}
 
 
@@ -310,11 +312,30 @@ def test_rolling_layouts_cut_sweeps_at_28q(pyqc):
     _layout_checks(roll, 28, 12, 4, ops)
 
 
-def test_rolling_ignored_with_tangents(pyqc):
+def test_rolling_layouts_with_tangents(pyqc):
+    """Derivative plans (generator terms for every parameter) under rolling
+    layouts: the layout invariants hold, fewer sweeps than fixed layouts,
+    and every tangent is spawned in a sweep that holds its generator."""
     from pyramaterised_amd import engine as E
     from pyramaterised_amd import templates as T
     qc = pyqc.PQC(16)
-    for l in T.clifford_HE_layers(4, 16):
+    for l in T.clifford_HE_layers(10, 16):
         qc.add_layer(l)
-    plan = E.Plan(16, qc.ops(True), tile_bits=12, low_bits=3, flags=L.PQS_PLAN_ROLLING)
-    assert not plan.rolling
+    ops = qc.ops(True)
+    fixed = E.Plan(16, ops, tile_bits=12, low_bits=3)
+    plan = E.Plan(16, ops, tile_bits=12, low_bits=3, flags=L.PQS_PLAN_ROLLING)
+    assert plan.rolling and not fixed.rolling
+    assert plan.n_sweeps < fixed.n_sweeps
+    _layout_checks(plan, 16, 12, 3, ops)
+    sweeps = plan.sweeps()
+    for t in range(qc.n_true_params):
+        s0 = plan.tangent_sweep(t)
+        gen = [i for i, o in enumerate(ops) if o.kind == L.OP_GEN and o.tangent == t]
+        assert gen and all(i in sweeps[s0][1] for i in gen), t
+    # the Python layer picks rolling derivative plans while the ping-pong
+    # rows fit the cap
+    assert qc.plan_options(True)["flags"] & L.PQS_PLAN_ROLLING
+    big = pyqc.PQC(22)
+    for l in T.clifford_HE_layers(40, 22):
+        big.add_layer(l)
+    assert not big.plan_options(True).get("flags", 0) & L.PQS_PLAN_ROLLING
