@@ -43,6 +43,7 @@ class PQC:
         self.jit = jit
         self._angles = None
         self._plans = {}
+        self._plan_uses = {}
         self.set_gates()
 
     # ------------------------------------------------------------ set-up
@@ -104,6 +105,7 @@ class PQC:
                 k += g.param_count
         self.n_true_params = k
         self._plans = {}
+        self._plan_uses = {}
 
     # ------------------------------------------------------------ parameters
     def get_params(self):
@@ -167,11 +169,20 @@ class PQC:
                     out += g.gen_ops(slot + w, slot, which=w + 1)
         return out
 
+    # with jit="auto", a plan used this many times gets specialised kernels
+    # even for small batches (repeated QFIMs, training loops)
+    JIT_AFTER_USES = 3
+
     def plan(self, with_generators=False, dtype=None, batch=1):
         dt = self.dtype if dtype is None else E.dtype_code(dtype)
         key = (with_generators, dt)
         p = self._plans.get(key)
+        uses = self._plan_uses[key] = self._plan_uses.get(key, 0) + 1
         want = self._want_jit(batch)
+        if (p is not None and not want and not getattr(p, "jitted", False) and self.jit == "auto"
+                and uses >= self.JIT_AFTER_USES and self._jit_env() is None):
+            p.jit()  # the same plan, specialised in place (no tile-shape tuning)
+            return p
         if p is not None and (getattr(p, "jitted", False) or not want):
             return p
         ops = self.ops(with_generators)
@@ -224,9 +235,13 @@ class PQC:
             return False
         return not with_generators and not any(self.plan_opts.values())
 
-    def _want_jit(self, batch):
+    @staticmethod
+    def _jit_env():
         import os
-        env = os.environ.get("PQS_JIT")
+        return os.environ.get("PQS_JIT")
+
+    def _want_jit(self, batch):
+        env = self._jit_env()
         if env is not None:
             return env not in ("0", "", "false", "False")
         if self.jit == "auto":
