@@ -1070,6 +1070,56 @@ __global__ __launch_bounds__(RB) void k_mw_final(const double *__restrict__ part
 // block, K split over blockIdx.z, partials reduced in order afterwards.
 constexpr int GT = 16, GK = 16;
 
+// One-row Gram (a GEMV): out[z][q] = sum over k in chunk z of conj(a_k) b_q,k
+// for 8 rows q per block; a is read once per block and stays in L2 across
+// blocks, each b row once.  Wave shuffles, then LDS across the 4 waves.
+constexpr int BV_ROWS = 8;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_bvec_partial(const typename CT<R>::V *__restrict__ A,
+                                                      const typename CT<R>::V *__restrict__ B, int64_t ldb,
+                                                      int64_t nn, int64_t dim, int64_t kchunk,
+                                                      double2 *__restrict__ part) {
+  __shared__ double red[4][BV_ROWS][2];
+  const int64_t q0 = int64_t(blockIdx.x) * BV_ROWS;
+  const int64_t k0 = int64_t(blockIdx.y) * kchunk;
+  const int64_t k1 = k0 + kchunk < dim ? k0 + kchunk : dim;
+  const int nr = nn - q0 < BV_ROWS ? int(nn - q0) : BV_ROWS;
+  double ar[BV_ROWS], ai[BV_ROWS];
+#pragma unroll
+  for (int r = 0; r < BV_ROWS; ++r) ar[r] = ai[r] = 0.0;
+  for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+    const auto x = A[k];
+#pragma unroll
+    for (int r = 0; r < BV_ROWS; ++r)
+      if (r < nr) {
+        const auto y = B[(q0 + r) * ldb + k];
+        ar[r] += double(x.x) * y.x + double(x.y) * y.y;
+        ai[r] += double(x.x) * y.y - double(x.y) * y.x;
+      }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < BV_ROWS; ++r) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ar[r] += __shfl_xor(ar[r], o);
+      ai[r] += __shfl_xor(ai[r], o);
+    }
+    if (lane == 0) {
+      red[w][r][0] = ar[r];
+      red[w][r][1] = ai[r];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nr) {
+    const int r = threadIdx.x;
+    part[int64_t(blockIdx.y) * nn + q0 + r] =
+        double2{red[0][r][0] + red[1][r][0] + red[2][r][0] + red[3][r][0],
+                red[0][r][1] + red[1][r][1] + red[2][r][1] + red[3][r][1]};
+  }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_gram_partial(const typename CT<R>::V *__restrict__ A, int64_t lda,
                                                       int64_t m, const typename CT<R>::V *__restrict__ B,
@@ -1967,6 +2017,15 @@ namespace {
 
 // K split of a Gram product: enough blocks to fill the device.
 void gram_split(const pqs_ctx *ctx, int64_t m, int64_t nn, int64_t dim, int64_t &ks, int64_t &kchunk) {
+  if (m == 1) {  // k_bvec_partial: BV_ROWS rows per block
+    const int64_t tn = (nn + BV_ROWS - 1) / BV_ROWS;
+    ks = std::max<int64_t>(1, (4 * int64_t(ctx->n_cu) + tn - 1) / tn);
+    ks = std::min<int64_t>(ks, (dim + 1023) / 1024);
+    ks = std::min<int64_t>(ks, 65535);
+    kchunk = (dim + ks - 1) / ks;
+    ks = (dim + kchunk - 1) / kchunk;
+    return;
+  }
   const int64_t tm = (m + GT - 1) / GT, tn = (nn + GT - 1) / GT;
   ks = std::max<int64_t>(1, (4 * int64_t(ctx->n_cu) + tm * tn - 1) / (tm * tn));
   ks = std::min<int64_t>(ks, (dim + 255) / 256);
@@ -1987,6 +2046,21 @@ int gram_impl(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m
               int64_t nn, int64_t dim, double2 *out, double2 *part, hipStream_t st) {
   int64_t ks, kchunk;
   gram_split(ctx, m, nn, dim, ks, kchunk);
+  if (m == 1) {
+    const int64_t tb = (nn + BV_ROWS - 1) / BV_ROWS;
+    if (tb > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "Gram too large");
+    dim3 grid((unsigned)tb, (unsigned)ks);
+    if (dtype == PQS_C128)
+      hipLaunchKernelGGL(k_bvec_partial<double>, grid, dim3(256), 0, st, (const double2 *)a, (const double2 *)b, ldb,
+                         nn, dim, kchunk, part);
+    else
+      hipLaunchKernelGGL(k_bvec_partial<float>, grid, dim3(256), 0, st, (const float2 *)a, (const float2 *)b, ldb,
+                         nn, dim, kchunk, part);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(nn, 256)), dim3(256), 0, st, part, nn, ks, out);
+    HIPCHECK(hipGetLastError());
+    return PQS_OK;
+  }
   const int64_t tm = (m + GT - 1) / GT, tn = (nn + GT - 1) / GT;
   if (tm > 0x7fffffff || tn > 65535) return fail(PQS_ERR_UNSUPPORTED, "Gram too large");
   dim3 grid((unsigned)tm, (unsigned)tn, (unsigned)ks);
