@@ -63,3 +63,38 @@ def test_small_states_and_empty_circuits(pyqc, n):
     z = np.zeros(2 ** n)
     z[0] = 1
     assert np.max(np.abs(out - z)) == 0
+
+
+@pytest.mark.parametrize("n,dtype,tol", [(30, "complex128", 1e-11), (31, "complex64", 2e-4)])
+def test_largest_states_round_trip(pyqc, n, dtype, tol):
+    """A 16 GiB state (30 qubits c128, 31 qubits c64): a brickwork circuit
+    followed by its inverse (the same gates in reverse order, R_y angles
+    negated; CZ is its own inverse) returns |0...0>.  Size-independent:
+    amplitude 0 is 1 and every other amplitude 0, to rounding.  Covers
+    64-bit addressing and the largest tiles' index maps."""
+    import torch
+    from pyramaterised_amd import engine as E
+    print(f"case n={n} dtype={dtype}", flush=True)
+    layers = 6
+    gates, inv = [], []
+    for lay in range(layers):
+        for q in range(n):
+            gates.append(("R_y", q))
+        for q in range(lay % 2, n - 1, 2):
+            gates.append(("CZ", q))
+    qc = pyqc.PQC(n, dtype=dtype)
+    fwd = [pyqc.R_y(q, n) if k == "R_y" else pyqc.CZ([q, q + 1], n) for k, q in gates]
+    back = [pyqc.R_y(q, n) if k == "R_y" else pyqc.CZ([q, q + 1], n) for k, q in reversed(gates)]
+    qc.add_layer(fwd + back)
+    m = sum(1 for k, _ in gates if k == "R_y")
+    th = np.random.default_rng(n).random(m) * 2 * np.pi
+    params = np.concatenate([th, -th[::-1]])
+    plan = qc.plan()
+    out = torch.empty((1, 2 ** n), dtype=E.CDTYPE[plan.dtype], device="cuda")
+    E.run(plan, torch.as_tensor(params, device="cuda"), qc.init(), out=out)
+    a0 = complex(out[0, 0].item())
+    assert abs(a0 - 1) < tol, a0
+    out[0, 0] = 0
+    assert float(out.abs().max().item()) < tol
+    del out
+    torch.cuda.empty_cache()
