@@ -279,10 +279,10 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     o << "    }\n  };\n";
     const bool persist = (p.flags & PQS_PLAN_JIT_PERSISTENT) != 0;
     // Direct load / store: the first (last) phase's registers come from (go
-    // to) HBM without staging the tile in LDS; its lane bits 0-3 are the
-    // tile bits at positions 0-3, so each 16 lanes touch 16 contiguous
-    // amplitudes.  Otherwise the tile is staged with thread bit i = tile bit
-    // i and register bit b = tile bit kt + b.
+    // to) HBM without staging the tile in LDS; the planner made the tile
+    // bits at read (write) positions 0 and 1 lane bits of that phase, so
+    // every 4 lanes touch 64 contiguous bytes.  Otherwise the tile is staged
+    // with thread bit i = tile bit i and register bit b = tile bit kt + b.
     const bool dload = ds.direct_load && !persist && ds.phase_end > ds.phase_begin;
     const bool dstore = ds.direct_store && !persist && ds.phase_end > ds.phase_begin;
     int lthr[kMaxTileBits], lreg[kRegBits];  // global positions of the load's thread / register bits

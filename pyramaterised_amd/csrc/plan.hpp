@@ -89,8 +89,8 @@ struct DevSweep {
   int32_t n;  // qubits of the state
   int32_t sperm[kMaxTileBits], wpos[kMaxTileBits];
   int32_t cwpos[kMaxQubits];
-  // Lane bits 0-3 of the first (last) register phase are the tile bits read
-  // (written) at positions 0-3, so the specialised kernel loads (stores)
+  // The tile bits read (written) at positions 0 and 1 are lane bits of the
+  // first (last) register phase, so the specialised kernel loads (stores)
   // that phase's registers directly, without staging the tile in LDS.
   int32_t direct_load, direct_store;
 };
