@@ -455,12 +455,6 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     ds.phase_begin = int(p.dev_phases.size());
     int n_coef = 0;
     std::vector<int> exec_order;
-    // Wave bits: thread bits 6.. select the wave of a workgroup.  A run of
-    // register phases that keeps the same tile bits there (none of them a
-    // register bit) only permutes amplitudes within each wave, so the
-    // specialised kernels exchange through LDS without a workgroup barrier
-    // inside the run.  Per run, the wave bits are the tile bits needed
-    // latest.
     // tile bit read / written at positions 0-3 (-1 if none); the first /
     // last register phase wants them as lane bits
     int rbit[4] = {-1, -1, -1, -1}, wbit[4] = {-1, -1, -1, -1};
@@ -471,6 +465,12 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     }
     // direct loads / stores (specialised kernels; not with persistent blocks)
     const bool direct_ok = !(p.flags & (PQS_PLAN_JIT_PERSISTENT | PQS_PLAN_NO_DIRECT));
+    // Wave bits: thread bits 6.. select the wave of a workgroup.  A run of
+    // register phases that keeps the same tile bits there (none of them a
+    // register bit) only permutes amplitudes within each wave, so the
+    // specialised kernels exchange through LDS without a workgroup barrier
+    // inside the run.  Per run, the wave bits are the tile bits needed
+    // latest.
     const int nwb = std::max(0, ds.k - kRegBits - 6);
     std::vector<int> phase_w(phases.size(), 0);  // tile-bit mask of the wave bits
     if (p.reg_exec && nwb > 0 && !(p.flags & PQS_PLAN_NO_WAVE_LOCAL)) {
