@@ -424,10 +424,11 @@ def main():
         "metric": "gates/s + HBM GB/s frac; batched+adjoint vs plain QFIM/expr speedup @1e-12",
         "value": head["gates_per_s"], "unit": "gates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "complex128 (f64)", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batched expressibility: 16-qubit HEA (R_y,R_z + CNOT CHAIN) x 20 layers, "
                                "940 gates, 640 params, 5000 pairs (10,000 states), 75-bin KL vs Haar(2^16)",
-                   "n_qubits": 16, "layers": 20, "states": head["S"], "pairs": head["S"] // 2},
+                   "n_qubits": 16, "layers": 20, "states": head["S"], "pairs": head["S"] // 2,
+                   "amplitudes": "complex128"},
         # The headline's sweeps hold 10,000 resident 1 MiB states and run
         # ~10 register phases per LDS tile: FP64 VALU + LDS bound, so the
         # compute roofline applies (FP64 vector peak = FP64 matrix peak).
