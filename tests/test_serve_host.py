@@ -30,3 +30,24 @@ def test_request_validation():
         SV.Request(0, 4, 2, "other", np.zeros(16))
     with pytest.raises(ValueError):
         SV.Request(0, 4, 2, SV.STATE_MW, np.zeros(15))
+
+
+def test_request_bytes_counts_rolling_jacobian_rows():
+    """A QFIM request needs its Jacobian (P rows), psi and a spare state,
+    plus P ping-pong rows when its derivative plan uses rolling layouts,
+    which the circuit layer picks while those rows fit its cap."""
+    import types
+    from pyramaterised_amd import serve as SV
+    from pyramaterised_amd import templates as T
+    from pyramaterised_amd.circuit import PQC
+    stub = types.SimpleNamespace(amp=16)
+    for n, depth in ((10, 4), (20, 40)):
+        P = 2 * n * depth
+        req = SV.Request(0, n, depth, SV.QFIM, np.zeros(P), 0.0)
+        qc = PQC(n)
+        for l in T.clifford_HE_layers(depth, n):
+            qc.add_layer(l)
+        assert qc.n_true_params == P
+        rows = P + 2 + (P if qc.rolling_jacobian() else 0)
+        assert SV.Server.request_bytes(stub, req) == 16 * 2 ** n * rows + 48 * P * P
+    assert PQC(10).rolling_jacobian() is True
