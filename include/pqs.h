@@ -106,15 +106,21 @@ enum {
                                    / last register phase loads / stores its
                                    registers directly when the tile bits at
                                    positions 0 and 1 are lane bits there) */
-  PQS_PLAN_ROLLING = 8          /* rolling layouts: each sweep writes the state
+  PQS_PLAN_ROLLING = 8,         /* rolling layouts: each sweep writes the state
                                    with the qubit -> bit-position map under which
                                    the next sweep's tile is one contiguous block;
                                    low_bits is then the least overlap kept between
                                    consecutive tiles (contiguous write runs), not
                                    a fixed set of qubits.  Runs go out of place
                                    (ping-pong workspace of one state per
-                                   instance).  Ignored for plans with tangents
-                                   and for tiles below 10 qubits. */
+                                   instance).  Ignored for tiles below 10
+                                   qubits. */
+  PQS_PLAN_GREEDY_SWEEPS = 64   /* rolling plans: take every sweep's tile from
+                                   the greedy pass over the op order (default:
+                                   also try tiles seeded with a window of
+                                   consecutive qubits and keep the one after
+                                   which the greedy pass needs the fewest
+                                   sweeps) */
 };
 
 /* One instance of a batched run. */

@@ -191,9 +191,10 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
   // the tile moves freely over the qubits.
   p.rolling = (o.flags & PQS_PLAN_ROLLING) && n > k && k >= kRegBits + 6;
 
-  // One sweep: greedy over the remaining ops with `want` in the tile first;
-  // if `want` leaves no room for the next op, without it.
-  auto plan_one = [&](qmask want, qmask &T, std::vector<int> &run, std::vector<int> &rest) {
+  // One sweep: greedy over the ops `rem` with `want` in the tile first; if
+  // `want` leaves no room for the next op, without it.
+  auto plan_one = [&](const std::vector<int> &rem, qmask want, qmask &T, std::vector<int> &run,
+                      std::vector<int> &rest) {
     run.clear();
     for (int attempt = 0; attempt < 2 && run.empty(); ++attempt) {
       T = (n <= k) ? all : (attempt == 0 ? want : 0);
@@ -201,7 +202,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
       rest.clear();
       qmask blocked_any = 0, blocked_nd = 0;
       int n_angle = 0;
-      for (int ei : remaining) {
+      for (int ei : rem) {
         const ExecOp &e = p.exec[ei];
         qmask touched = e.active | e.diag;
         bool dep = (touched & blocked_nd) || (e.active & blocked_any);
@@ -223,7 +224,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
           blocked_nd |= e.active;
         }
       }
-      if (remaining.empty()) break;
+      if (rem.empty()) break;
     }
   };
   // qubits of `from` ordered by their first non-diagonal use in `ops_left`
@@ -240,28 +241,28 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
       if (((from & ~seen) >> q) & 1) order.push_back(q);
     return order;
   };
-
-  qmask prevT = 0;
-  while (!remaining.empty() || p.sweeps.empty()) {
-    qmask T = 0;
-    std::vector<int> run, rest;
-    qmask want = forced;
-    if (p.rolling && !p.sweeps.empty()) {
-      want = 0;  // carry the `low` previous-tile qubits needed soonest
-      for (int q : by_next_use(prevT, remaining)) {
-        if (popc(want) >= low) break;
-        want |= bitq(q);
-      }
+  // the `low` previous-tile qubits needed soonest (kept in the next tile)
+  auto carry_of = [&](qmask prevT, const std::vector<int> &rem) {
+    qmask want = 0;
+    for (int q : by_next_use(prevT, rem)) {
+      if (popc(want) >= low) break;
+      want |= bitq(q);
     }
-    plan_one(want, T, run, rest);
-    if (run.empty() && !remaining.empty()) return fail(PQS_ERR_UNSUPPORTED, "planner made no progress");
-    if (p.rolling && rest.empty() && (T & forced) != forced && popc(want | forced) <= k) {
+    return want;
+  };
+  // One sweep over `rem` after tile prevT, seeded with `want`: the greedy
+  // pass, the last sweep's layout rule and the fill of the tile to k qubits.
+  auto sweep_from = [&](qmask want, qmask prevT, const std::vector<int> &rem, qmask &T,
+                        std::vector<int> &run, std::vector<int> &rest) {
+    plan_one(rem, want, T, run, rest);
+    if (!run.empty() && p.rolling && rest.empty() && (T & forced) != forced &&
+        popc(want | forced) <= k) {
       // the last sweep writes the standard layout: keep its lowest qubits in
       // the tile (a shorter last sweep and one more are cheaper than a
       // scattered final write)
       qmask T2;
       std::vector<int> run2, rest2;
-      plan_one(want | forced, T2, run2, rest2);
+      plan_one(rem, want | forced, T2, run2, rest2);
       if (!run2.empty() && (T2 & forced) == forced) {
         T = T2;
         run.swap(run2);
@@ -281,6 +282,56 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     // Fill the tile to k qubits with the lowest free positions.
     for (int b = 0; b < n && popc(T) < k; ++b)
       if (!(T & bitq(n - 1 - b))) T |= bitq(n - 1 - b);
+  };
+  // Sweeps the plain greedy pass needs for `rem` after tile prevT.
+  auto greedy_count = [&](qmask prevT, std::vector<int> rem) {
+    int count = 0;
+    std::vector<int> run, rest;
+    while (!rem.empty()) {
+      qmask T;
+      sweep_from(carry_of(prevT, rem), prevT, rem, T, run, rest);
+      if (run.empty()) return 1 << 30;
+      prevT = T;
+      rem.swap(rest);
+      ++count;
+    }
+    return count;
+  };
+  // Window search (rolling plans): besides the greedy tile, try tiles seeded
+  // with the carried qubits plus a window of k - low consecutive qubits, and
+  // keep the one after which the greedy pass needs the fewest sweeps.  On
+  // nearest-neighbour circuits the greedy tile follows the op order and
+  // leaves ragged light-cone fronts; a window lets a sweep run a deep
+  // light-cone block of its qubits.  The greedy tile wins ties.
+  const bool search = p.rolling && !(o.flags & PQS_PLAN_GREEDY_SWEEPS) && low < k;
+  const int win = k - low;
+
+  qmask prevT = 0;
+  while (!remaining.empty() || p.sweeps.empty()) {
+    const bool first = p.sweeps.empty();
+    qmask T = 0;
+    std::vector<int> run, rest;
+    const qmask want = (p.rolling && !first) ? carry_of(prevT, remaining) : forced;
+    sweep_from(want, prevT, remaining, T, run, rest);
+    if (run.empty() && !remaining.empty()) return fail(PQS_ERR_UNSUPPORTED, "planner made no progress");
+    if (search && !rest.empty()) {
+      int best = greedy_count(T, rest);
+      qmask T2;
+      std::vector<int> run2, rest2;
+      for (int s0 = 0; s0 + win <= n; ++s0) {
+        const qmask W = want | (((qmask(1) << win) - 1) << s0);
+        if (popc(W) > k) continue;
+        sweep_from(W, prevT, remaining, T2, run2, rest2);
+        if (run2.empty() || (!first && popc(T2 & prevT) < low)) continue;
+        const int c = greedy_count(T2, rest2);
+        if (c < best) {
+          best = c;
+          T = T2;
+          run.swap(run2);
+          rest.swap(rest2);
+        }
+      }
+    }
     Sweep sw;
     for (int b = 0; b < n; ++b)
       if (T & bitq(n - 1 - b)) sw.tile_qubits.push_back(n - 1 - b);

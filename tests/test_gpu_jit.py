@@ -277,7 +277,8 @@ def test_direct_load_store_match_staged_tiles(pyqc, dtype, tol):
 def test_rolling_all_reg_kinds_and_inplace_input(pyqc):
     """All register op kinds under rolling layouts; a vector input state;
     and the C ABI's in-place form (init 0, no source: the state buffer is
-    the input) for plans with an odd and an even number of sweeps."""
+    the input) for plans with an odd and an even number of sweeps, from the
+    window sweep search and from the plain greedy sweeps."""
     import ctypes as C
     import torch
     from pyramaterised_amd import _lib as L
@@ -287,7 +288,8 @@ def test_rolling_all_reg_kinds_and_inplace_input(pyqc):
     kinds1 = ["R_x", "R_y", "R_z", "H", "T", "S", "X"]
     kinds2 = ["CNOT", "CZ", "R_zz"]
     parities = set()
-    for depth in (2, 3, 7, 8):
+    R, G = L.PQS_PLAN_ROLLING, L.PQS_PLAN_GREEDY_SWEEPS
+    for depth, flags in ((2, R), (4, R), (7, R), (8, R | G)):
         srng = np.random.default_rng(100 + depth)
         spec = []
         for _ in range(depth):
@@ -298,7 +300,7 @@ def test_rolling_all_reg_kinds_and_inplace_input(pyqc):
                 spec.append((kinds2[srng.integers(len(kinds2))], (a, b)))
         pk = [getattr(pyqc, nm)(list(q) if isinstance(q, tuple) else q, n) for nm, q in spec]
         og = [getattr(OC, nm)(q, n) for nm, q in spec]
-        qc = _build(pyqc, n, [pk], True, tile_bits=10, low_bits=3, flags=L.PQS_PLAN_ROLLING)
+        qc = _build(pyqc, n, [pk], True, tile_bits=10, low_bits=3, flags=flags)
         plan = qc.plan()
         assert plan.rolling and plan.jitted
         parities.add(plan.n_sweeps % 2)

@@ -312,6 +312,54 @@ def test_rolling_layouts_cut_sweeps_at_28q(pyqc):
     _layout_checks(roll, 28, 12, 4, ops)
 
 
+def test_window_search_never_adds_sweeps(pyqc):
+    """The window sweep search keeps a window tile only when the greedy pass
+    then needs fewer sweeps for the rest, so it never uses more sweeps than
+    the plain greedy plan; on nearest-neighbour circuits it uses fewer.  Its
+    plans keep the rolling-layout invariants and an op order equivalent to
+    program order."""
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    R, G = L.PQS_PLAN_ROLLING, L.PQS_PLAN_GREEDY_SWEEPS
+
+    def counts(qc, k, low):
+        ops = qc.ops()
+        a = E.Plan(qc.n_qubits, ops, tile_bits=k, low_bits=low, flags=R)
+        b = E.Plan(qc.n_qubits, ops, tile_bits=k, low_bits=low, flags=R | G)
+        _layout_checks(a, qc.n_qubits, k, low, ops)
+        return a, b
+
+    bw = pyqc.PQC(28)
+    for l in T.ry_cz_brickwork_layers(30, 28):
+        bw.add_layer(l)
+    a, b = counts(bw, 12, 4)
+    assert (a.n_sweeps, b.n_sweeps) == (15, 17)
+    he = pyqc.PQC(20)
+    for l in T.clifford_HE_layers(40, 20):
+        he.add_layer(l)
+    a, b = counts(he, 12, 4)
+    assert a.n_sweeps <= 17 and b.n_sweeps >= 38
+    he16 = pyqc.PQC(16)
+    for l in T.clifford_HE_layers(20, 16):
+        he16.add_layer(l)
+    a, b = counts(he16, 11, 3)
+    assert a.n_sweeps < b.n_sweeps
+    ops = he16.ops()
+    order = [i for _, idx in a.sweeps() for i in idx]
+    assert sorted(order) == list(range(len(ops)))
+    params = np.random.default_rng(9).random(he16.n_true_params) * 2 * np.pi
+    x = simulate_ops(ops, range(len(ops)), 16, params)
+    y = simulate_ops(ops, order, 16, params)
+    assert np.max(np.abs(x - y)) < 1e-12
+    for seed in range(4):
+        rng = np.random.default_rng(50 + seed)
+        spec = [s for s in random_spec(18, 5, rng) if s[0] != "shared_Rx"]
+        qc = pyqc.PQC(18)
+        qc.add_layer(build_pkg(spec, 18, pyqc))
+        a, b = counts(qc, 12, 3)
+        assert a.n_sweeps <= b.n_sweeps
+
+
 def test_rolling_layouts_with_tangents(pyqc):
     """Derivative plans (generator terms for every parameter) under rolling
     layouts: the layout invariants hold, fewer sweeps than fixed layouts,
