@@ -467,10 +467,15 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         size_t j = i;
         while (j < sw.ops.size() && reg_capable(sw.ops[j])) ++j;
         std::vector<int> rem(sw.ops.begin() + i, sw.ops.begin() + j);
-        while (!rem.empty()) {
-          qmask R = 0, blocked_any = 0, blocked_nd = 0;
-          std::vector<int> run, rest;
-          for (int ei : rem) {
+        // One register phase over `rem`, its register set grown from `seed`
+        // in op order with the sweep's dependency rule.
+        auto phase_from = [&](const std::vector<int> &from, qmask seed, qmask &R, std::vector<int> &run,
+                              std::vector<int> &rest) {
+          R = seed;
+          run.clear();
+          rest.clear();
+          qmask blocked_any = 0, blocked_nd = 0;
+          for (int ei : from) {
             const ExecOp &e = p.exec[ei];
             qmask touched = e.active | e.diag;
             bool dep = (touched & blocked_nd) || (e.active & blocked_any);
@@ -491,7 +496,47 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
               blocked_nd |= e.active;
             }
           }
+        };
+        // Phases the plain greedy pass needs for `from`.
+        auto greedy_phases = [&](std::vector<int> from) {
+          int count = 0;
+          qmask R;
+          std::vector<int> run, rest;
+          while (!from.empty()) {
+            phase_from(from, 0, R, run, rest);
+            if (run.empty()) return 1 << 30;
+            from.swap(rest);
+            ++count;
+          }
+          return count;
+        };
+        // Window phase search (as for sweeps): also try register sets seeded
+        // with kRegBits consecutive tile qubits and keep the phase after
+        // which the greedy pass needs the fewest phases; greedy wins ties.
+        const bool phase_search = !(p.flags & PQS_PLAN_GREEDY_PHASES);
+        while (!rem.empty()) {
+          qmask R;
+          std::vector<int> run, rest;
+          phase_from(rem, 0, R, run, rest);
           if (run.empty()) return fail(PQS_ERR_UNSUPPORTED, "phase planner made no progress");
+          if (phase_search && !rest.empty()) {
+            int best = greedy_phases(rest);
+            qmask R2;
+            std::vector<int> run2, rest2;
+            for (int q0 = 0; q0 + kRegBits <= n; ++q0) {
+              const qmask W = ((qmask(1) << kRegBits) - 1) << q0;
+              if ((W & tmask) != W) continue;
+              phase_from(rem, W, R2, run2, rest2);
+              if (run2.empty()) continue;
+              const int c = greedy_phases(rest2);
+              if (c < best) {
+                best = c;
+                R = R2;
+                run.swap(run2);
+                rest.swap(rest2);
+              }
+            }
+          }
           phases.push_back({false, run, R});
           rem.swap(rest);
         }

@@ -360,6 +360,46 @@ def test_window_search_never_adds_sweeps(pyqc):
         assert a.n_sweeps <= b.n_sweeps
 
 
+def _phase_counts(plan):
+    """Register phases per sweep, counted in the specialised kernel sources
+    (each register phase declares its pending-flip flags once)."""
+    return [src.decode().count("int xf0 = 0") for src in plan.jit_sweep_sources()]
+
+
+def test_phase_search_never_adds_phases(pyqc):
+    """The register-phase window search keeps a seeded register set only
+    when the greedy pass then needs fewer phases for the rest of the run, so
+    no sweep gets more phases than with greedy phases; on the HEA and
+    brickwork circuits it removes 14-19% of them."""
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    R, GP = L.PQS_PLAN_ROLLING, L.PQS_PLAN_GREEDY_PHASES
+
+    def both(qc, k, low):
+        a = E.Plan(qc.n_qubits, qc.ops(), tile_bits=k, low_bits=low, flags=R)
+        b = E.Plan(qc.n_qubits, qc.ops(), tile_bits=k, low_bits=low, flags=R | GP)
+        assert a.n_sweeps == b.n_sweeps
+        pa, pb = _phase_counts(a), _phase_counts(b)
+        assert all(x <= y for x, y in zip(pa, pb)), (pa, pb)
+        return sum(pa), sum(pb)
+
+    he = pyqc.PQC(16)
+    for l in T.clifford_HE_layers(20, 16):
+        he.add_layer(l)
+    assert both(he, 12, 4) == (60, 70)
+    bw = pyqc.PQC(28)
+    for l in T.ry_cz_brickwork_layers(30, 28):
+        bw.add_layer(l)
+    assert both(bw, 12, 4) == (68, 84)
+    for seed in range(3):
+        rng = np.random.default_rng(70 + seed)
+        spec = [s for s in random_spec(14, 5, rng) if s[0] != "shared_Rx"]
+        qc = pyqc.PQC(14)
+        qc.add_layer(build_pkg(spec, 14, pyqc))
+        a, b = both(qc, 11, 3)
+        assert a <= b
+
+
 def test_rolling_layouts_with_tangents(pyqc):
     """Derivative plans (generator terms for every parameter) under rolling
     layouts: the layout invariants hold, fewer sweeps than fixed layouts,
