@@ -41,9 +41,13 @@ def sync():
     torch.cuda.synchronize()
 
 
-def build_c2(pyqc, n=16, p=20):
+def build_c2(pyqc, n=16, p=20, shape=None):
+    """The headline circuit; ``shape`` = (tile bits, kept bits) fixes a
+    rolling-layout tile shape instead of the tuned one."""
+    from pyramaterised_amd import _lib as L
     from pyramaterised_amd import templates as T
-    qc = pyqc.PQC(n)
+    kw = {} if shape is None else dict(tile_bits=shape[0], low_bits=shape[1], flags=L.PQS_PLAN_ROLLING)
+    qc = pyqc.PQC(n, **kw)
     for l in T.clifford_HE_layers(p, n):
         qc.add_layer(l)
     return qc
@@ -90,7 +94,7 @@ class EventTimer:
 def headline(pyqc, args):
     import torch
     from pyramaterised_amd import engine as E
-    qc = build_c2(pyqc)
+    qc = build_c2(pyqc, shape=args.shape)
     P = qc.n_true_params
     n_pairs = args.pairs
     S = 2 * n_pairs
@@ -320,15 +324,16 @@ def leg_serving(args):
 
 def traffic_per_launch(head):
     """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE /
-    WRITE_SIZE on this workload, scaled by the same counters' reading for a
-    plain copy of known size (profiles/traffic_headline.json); None when that
-    measurement is absent."""
+    WRITE_SIZE on this workload with the same tile shape and sweep count as
+    this run's plan, scaled by the same counters' reading for a plain copy
+    of known size (profiles/traffic_headline.json, one entry per plan
+    shape); None when no measurement matches."""
     path = os.path.join(ROOT, "profiles", "traffic_headline.json")
     try:
         with open(path) as f:
-            t = json.load(f)
+            t = json.load(f)["plans"][f"{head['tile_bits']}x{head['n_sweeps']}"]
         return float(t["bytes_per_launch"]) * head["S"] / float(t["states"])
-    except (OSError, ValueError, KeyError):
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
@@ -373,6 +378,9 @@ def main():
                     help="extra legs after the headline (comma list of 28q, qfim12, serving, speedups)")
     ap.add_argument("--only-legs", action="store_true",
                     help="skip the headline and the CPU baseline; print the legs' JSON only (for profiling a leg)")
+    ap.add_argument("--shape", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
+                    help="diagnostic: fix the headline's tile shape as 'TILE_BITS,KEPT_BITS' (rolling layouts) "
+                         "instead of tuning it")
     ap.add_argument("--calib", action="store_true",
                     help="first copy 2 GiB (complex128, 2^27 elements) three times: a plain copy of known size "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE readings")

@@ -182,9 +182,6 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
   for (int b = 0; b < low; ++b) forced |= bitq(n - 1 - b);
   qmask all = (n == 64) ? ~qmask(0) : ((qmask(1) << n) - 1);
 
-  std::vector<int> remaining(p.exec.size());
-  for (size_t i = 0; i < remaining.size(); ++i) remaining[i] = int(i);
-
   // Rolling layouts: the qubits kept from one tile to the next (at least
   // `low` of them) become the lowest bit positions of the layout in between,
   // so both the contiguous tile reads and the write runs are preserved while
@@ -283,63 +280,89 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     for (int b = 0; b < n && popc(T) < k; ++b)
       if (!(T & bitq(n - 1 - b))) T |= bitq(n - 1 - b);
   };
-  // Sweeps the plain greedy pass needs for `rem` after tile prevT.
-  auto greedy_count = [&](qmask prevT, std::vector<int> rem) {
+  // The greedy pass over `rem` after tile prevT, at most `horizon` sweeps
+  // ahead: the sweeps it needs (horizon + 1 if more) and the ops still left
+  // then.  Fewer sweeps is better, then fewer ops left.
+  struct Ahead {
+    int sweeps;
+    size_t left;
+    bool operator<(const Ahead &o) const { return sweeps != o.sweeps ? sweeps < o.sweeps : left < o.left; }
+  };
+  auto greedy_ahead = [&](int horizon, qmask prevT, std::vector<int> rem) {
     int count = 0;
     std::vector<int> run, rest;
     while (!rem.empty()) {
+      if (count == horizon) return Ahead{count + 1, rem.size()};
       qmask T;
       sweep_from(carry_of(prevT, rem), prevT, rem, T, run, rest);
-      if (run.empty()) return 1 << 30;
+      if (run.empty()) return Ahead{1 << 30, rem.size()};
       prevT = T;
       rem.swap(rest);
       ++count;
     }
-    return count;
+    return Ahead{count, 0};
   };
   // Window search (rolling plans): besides the greedy tile, try tiles seeded
   // with the carried qubits plus a window of k - low consecutive qubits, and
-  // keep the one after which the greedy pass needs the fewest sweeps.  On
-  // nearest-neighbour circuits the greedy tile follows the op order and
-  // leaves ragged light-cone fronts; a window lets a sweep run a deep
-  // light-cone block of its qubits.  The greedy tile wins ties.
+  // keep the one after which the greedy pass, looking `horizon` sweeps
+  // ahead, needs the fewest sweeps (then leaves the fewest ops); the greedy
+  // tile wins ties.  On nearest-neighbour circuits the greedy tile follows
+  // the op order and leaves ragged light-cone fronts; a window lets a sweep
+  // run a deep light-cone block of its qubits.  Short horizons gave the
+  // fewest sweeps (a full look-ahead was both slower and worse), so the
+  // partition is built for horizons 1, 2 and 3 and the one with the fewest
+  // sweeps is kept.
   const bool search = p.rolling && !(o.flags & PQS_PLAN_GREEDY_SWEEPS) && low < k;
   const int win = k - low;
-
-  qmask prevT = 0;
-  while (!remaining.empty() || p.sweeps.empty()) {
-    const bool first = p.sweeps.empty();
-    qmask T = 0;
-    std::vector<int> run, rest;
-    const qmask want = (p.rolling && !first) ? carry_of(prevT, remaining) : forced;
-    sweep_from(want, prevT, remaining, T, run, rest);
-    if (run.empty() && !remaining.empty()) return fail(PQS_ERR_UNSUPPORTED, "planner made no progress");
-    if (search && !rest.empty()) {
-      int best = greedy_count(T, rest);
-      qmask T2;
-      std::vector<int> run2, rest2;
-      for (int s0 = 0; s0 + win <= n; ++s0) {
-        const qmask W = want | (((qmask(1) << win) - 1) << s0);
-        if (popc(W) > k) continue;
-        sweep_from(W, prevT, remaining, T2, run2, rest2);
-        if (run2.empty() || (!first && popc(T2 & prevT) < low)) continue;
-        const int c = greedy_count(T2, rest2);
-        if (c < best) {
-          best = c;
-          T = T2;
-          run.swap(run2);
-          rest.swap(rest2);
+  auto partition = [&](int horizon, std::vector<Sweep> &out) {
+    out.clear();
+    std::vector<int> rem(p.exec.size());
+    for (size_t i = 0; i < rem.size(); ++i) rem[i] = int(i);
+    qmask prevT = 0;
+    while (!rem.empty() || out.empty()) {
+      const bool first = out.empty();
+      qmask T = 0;
+      std::vector<int> run, rest;
+      const qmask want = (p.rolling && !first) ? carry_of(prevT, rem) : forced;
+      sweep_from(want, prevT, rem, T, run, rest);
+      if (run.empty() && !rem.empty()) return false;
+      if (horizon > 0 && !rest.empty()) {
+        Ahead best = greedy_ahead(horizon, T, rest);
+        qmask T2;
+        std::vector<int> run2, rest2;
+        for (int s0 = 0; s0 + win <= n; ++s0) {
+          const qmask W = want | (((qmask(1) << win) - 1) << s0);
+          if (popc(W) > k) continue;
+          sweep_from(W, prevT, rem, T2, run2, rest2);
+          if (run2.empty() || (!first && popc(T2 & prevT) < low)) continue;
+          // a last sweep writes the standard layout: its tile must hold the
+          // lowest qubits (contiguous write runs)
+          if (rest2.empty() && (T2 & forced) != forced) continue;
+          const Ahead c = greedy_ahead(horizon, T2, rest2);
+          if (c < best) {
+            best = c;
+            T = T2;
+            run.swap(run2);
+            rest.swap(rest2);
+          }
         }
       }
+      Sweep sw;
+      for (int b = 0; b < n; ++b)
+        if (T & bitq(n - 1 - b)) sw.tile_qubits.push_back(n - 1 - b);
+      sw.ops = run;
+      out.push_back(sw);
+      prevT = T;
+      rem.swap(rest);
+      if (rem.empty()) break;
     }
-    Sweep sw;
-    for (int b = 0; b < n; ++b)
-      if (T & bitq(n - 1 - b)) sw.tile_qubits.push_back(n - 1 - b);
-    sw.ops = run;
-    p.sweeps.push_back(sw);
-    prevT = T;
-    remaining.swap(rest);
-    if (remaining.empty()) break;
+    return true;
+  };
+  if (!partition(0, p.sweeps)) return fail(PQS_ERR_UNSUPPORTED, "planner made no progress");
+  if (search) {
+    std::vector<Sweep> alt;
+    for (int horizon = 1; horizon <= 3; ++horizon)
+      if (partition(horizon, alt) && alt.size() < p.sweeps.size()) p.sweeps.swap(alt);
   }
 
   // Layouts: L_0 and L_S standard (pos = n-1-q); in between, L_s holds the

@@ -153,7 +153,7 @@ def tile_candidates(n, dtype_c):
     """
     R = L.PQS_PLAN_ROLLING
     if dtype_c == L.PQS_C128:
-        c = [(12, 4, R), (12, 3, R), (11, 3, R), (12, 4, 0)]
+        c = [(11, 3, R), (12, 4, R), (12, 3, R), (12, 4, 0)]
     else:
         c = [(12, 4, R), (13, 4, R), (11, 4, R), (13, 4, 0)]
     out = []
@@ -173,7 +173,8 @@ def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
     """Build the plan for each tile shape of ``tile_candidates``, specialise
     all of them, time evolutions of each on this GPU (random angles, up to
     2^26 amplitudes of the batch; the best of three interleaved rounds) and
-    return the fastest.  The choice is
+    return the first candidate within TUNE_TOLERANCE of the fastest.  The
+    choice is
     cached in memory and in the code-object cache directory, keyed by the op
     list, the dtype and the timed batch size."""
     cands = tile_candidates(n_qubits, dtype_c)
@@ -217,7 +218,10 @@ def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
             e.synchronize()
             times[i] = min(times[i], s.elapsed_time(e))
     del out
-    best = int(np.argmin(times))
+    # the first candidate (in list order) within TUNE_TOLERANCE of the
+    # fastest: near-ties do not flip the choice from box to box
+    fastest = min(times)
+    best = next(i for i, t in enumerate(times) if t <= fastest * (1 + TUNE_TOLERANCE))
     _tune_memo[key] = cands[best]
     try:
         os.makedirs(jit_cache_dir(), exist_ok=True)
@@ -230,6 +234,7 @@ def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
 
 
 _tune_memo = {}
+TUNE_TOLERANCE = 0.015
 
 
 def stream_ptr():

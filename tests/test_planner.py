@@ -313,11 +313,11 @@ def test_rolling_layouts_cut_sweeps_at_28q(pyqc):
 
 
 def test_window_search_never_adds_sweeps(pyqc):
-    """The window sweep search keeps a window tile only when the greedy pass
-    then needs fewer sweeps for the rest, so it never uses more sweeps than
-    the plain greedy plan; on nearest-neighbour circuits it uses fewer.  Its
-    plans keep the rolling-layout invariants and an op order equivalent to
-    program order."""
+    """The window sweep search builds partitions with look-aheads of 1, 2
+    and 3 sweeps and keeps one only when it has fewer sweeps than the plain
+    greedy partition, so it never uses more; on nearest-neighbour circuits it
+    uses far fewer.  Its plans keep the rolling-layout invariants and an op
+    order equivalent to program order."""
     from pyramaterised_amd import engine as E
     from pyramaterised_amd import templates as T
     R, G = L.PQS_PLAN_ROLLING, L.PQS_PLAN_GREEDY_SWEEPS
@@ -333,12 +333,12 @@ def test_window_search_never_adds_sweeps(pyqc):
     for l in T.ry_cz_brickwork_layers(30, 28):
         bw.add_layer(l)
     a, b = counts(bw, 12, 4)
-    assert (a.n_sweeps, b.n_sweeps) == (15, 17)
+    assert (a.n_sweeps, b.n_sweeps) == (11, 17)
     he = pyqc.PQC(20)
     for l in T.clifford_HE_layers(40, 20):
         he.add_layer(l)
     a, b = counts(he, 12, 4)
-    assert a.n_sweeps <= 17 and b.n_sweeps >= 38
+    assert a.n_sweeps <= 16 and b.n_sweeps >= 38
     he16 = pyqc.PQC(16)
     for l in T.clifford_HE_layers(20, 16):
         he16.add_layer(l)
@@ -370,7 +370,7 @@ def test_phase_search_never_adds_phases(pyqc):
     """The register-phase window search keeps a seeded register set only
     when the greedy pass then needs fewer phases for the rest of the run, so
     no sweep gets more phases than with greedy phases; on the HEA and
-    brickwork circuits it removes 14-19% of them."""
+    brickwork circuits it removes 14% of them."""
     from pyramaterised_amd import engine as E
     from pyramaterised_amd import templates as T
     R, GP = L.PQS_PLAN_ROLLING, L.PQS_PLAN_GREEDY_PHASES
@@ -390,7 +390,7 @@ def test_phase_search_never_adds_phases(pyqc):
     bw = pyqc.PQC(28)
     for l in T.ry_cz_brickwork_layers(30, 28):
         bw.add_layer(l)
-    assert both(bw, 12, 4) == (68, 84)
+    assert both(bw, 12, 4) == (65, 76)
     for seed in range(3):
         rng = np.random.default_rng(70 + seed)
         spec = [s for s in random_spec(14, 5, rng) if s[0] != "shared_Rx"]
