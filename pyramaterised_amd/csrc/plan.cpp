@@ -286,7 +286,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
   struct Ahead {
     int sweeps;
     size_t left;
-    bool operator<(const Ahead &o) const { return sweeps != o.sweeps ? sweeps < o.sweeps : left < o.left; }
+    bool operator<(const Ahead &b) const { return sweeps != b.sweeps ? sweeps < b.sweeps : left < b.left; }
   };
   auto greedy_ahead = [&](int horizon, qmask prevT, std::vector<int> rem) {
     int count = 0;
