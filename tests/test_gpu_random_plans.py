@@ -11,7 +11,7 @@ from tests.helpers import build_oracle, build_pkg, infidelity, random_spec
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("seed", range(14))
 def test_random_plan_matches_oracle(seed):
     import pyramaterised_amd as pyqc
     from pyramaterised_amd import _lib as L
@@ -23,7 +23,8 @@ def test_random_plan_matches_oracle(seed):
     tile = int(rng.integers(10, min(n, kmax) + 1))
     low = int(rng.integers(2, 5))
     flags = [0, L.PQS_PLAN_ROLLING, L.PQS_PLAN_ROLLING | L.PQS_PLAN_NO_DIRECT, L.PQS_PLAN_NO_WAVE_LOCAL,
-             L.PQS_PLAN_ROLLING | L.PQS_PLAN_LANES_ASCENDING, L.PQS_PLAN_NO_DIRECT][seed % 6]
+             L.PQS_PLAN_ROLLING | L.PQS_PLAN_LANES_ASCENDING, L.PQS_PLAN_NO_DIRECT,
+             L.PQS_PLAN_ROLLING | L.PQS_PLAN_GREEDY_SWEEPS | L.PQS_PLAN_GREEDY_PHASES][seed % 7]
     if seed % 2 == 0:  # only gates the specialised kernels take: every sweep is specialised
         depth = int(rng.integers(4, 9))
         spec = []
