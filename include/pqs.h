@@ -118,9 +118,9 @@ enum {
   PQS_PLAN_GREEDY_SWEEPS = 64,  /* rolling plans: take every sweep's tile from
                                    the greedy pass over the op order (default:
                                    also try tiles seeded with a window of
-                                   consecutive qubits and keep the one after
-                                   which the greedy pass needs the fewest
-                                   sweeps) */
+                                   consecutive qubits, scored by the greedy
+                                   pass 1, 2 or 3 sweeps ahead, and keep the
+                                   partition with the fewest sweeps) */
   PQS_PLAN_GREEDY_PHASES = 128  /* take every register phase's qubits from the
                                    greedy pass (default: also try register sets
                                    seeded with 4 consecutive tile qubits and
