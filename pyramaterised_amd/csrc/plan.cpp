@@ -364,6 +364,9 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     for (int horizon = 1; horizon <= 3; ++horizon)
       if (partition(horizon, alt) && alt.size() < p.sweeps.size()) p.sweeps.swap(alt);
   }
+  // A single sweep reads and writes the standard layout: it runs in place,
+  // with no second buffer.
+  if (p.sweeps.size() < 2) p.rolling = false;
 
   // Layouts: L_0 and L_S standard (pos = n-1-q); in between, L_s holds the
   // qubits kept from tile s-1 lowest, then the rest of tile s, then the

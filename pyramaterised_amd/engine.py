@@ -254,6 +254,7 @@ class Plan:
         if tile_bits == 0 and torch.cuda.is_available():
             tile_bits = max_tile_bits(self.dtype)
         self.n_ops = len(ops)
+        self.flags = int(flags)
         arr = (L.Op * max(1, len(ops)))(*ops)
         opts = L.PlanOpts(tile_bits, low_bits, block, self.dtype, flags)
         self.handle = C.c_void_p()
@@ -441,7 +442,11 @@ class BatchGraph:
             raise ValueError("out has the wrong shape, dtype or layout")
         code, src, tab = init.device_args(plan.dtype, dev)
         self._src, self._tab = src, tab
-        self.alt = torch.empty_like(self.out) if plan.rolling else None
+        # the second buffer of rolling layouts; a plan asked for them with more
+        # than one sweep may use them even where every layout it reports is
+        # standard, and a plan that does not ignores the buffer
+        need_alt = plan.rolling or (plan.flags & L.PQS_PLAN_ROLLING and plan.n_sweeps > 1)
+        self.alt = torch.empty_like(self.out) if need_alt else None
         nb = L.lib().pqs_batch_table_bytes(B)
         self.table = torch.empty(nb, dtype=torch.uint8, device=dev)
         check(L.lib().pqs_batch_table(ctx.handle, plan.handle, ptr(self.out), dim, B, ptr(self.params),

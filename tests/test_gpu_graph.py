@@ -44,3 +44,29 @@ def test_graph_replays_match_eager(pyqc, jit, flags, init_kind):
         torch.cuda.synchronize()
         ref = E.run(plan, prm, init)
         assert torch.max(torch.abs(got - ref)).item() < 1e-13, rep
+
+
+def test_graph_single_sweep_plan_with_rolling_flag(pyqc):
+    """A plan asked for rolling layouts whose ops fit one sweep runs in
+    place like a fixed-layout plan: the graph form needs no second buffer,
+    and replays equal the eager run."""
+    from pyramaterised_amd import _lib as L
+    from pyramaterised_amd import engine as E
+    n, S = 13, 4
+    layer = [pyqc.R_y(q, n) for q in range(2, n)] + [pyqc.CNOT([q, q + 1], n) for q in range(2, n - 1)]
+    qc = pyqc.PQC(n, tile_bits=11, low_bits=3, flags=L.PQS_PLAN_ROLLING, jit=True)
+    qc.add_layer(layer)
+    plan = qc.plan(batch=S)
+    assert plan.n_sweeps == 1 and not plan.rolling
+    rng = np.random.default_rng(51)
+    P = qc.n_true_params
+    init = E.Init.zero()
+    g = E.BatchGraph(plan, torch.as_tensor(rng.random((S, P)) * 2 * np.pi, device="cuda"), init)
+    assert g.alt is None
+    for rep in range(2):
+        prm = torch.as_tensor(rng.random((S, P)) * 2 * np.pi, device="cuda")
+        g.params.copy_(prm)
+        got = g.replay().clone()
+        torch.cuda.synchronize()
+        ref = E.run(plan, prm, init)
+        assert torch.max(torch.abs(got - ref)).item() < 1e-13, rep
