@@ -218,10 +218,7 @@ def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
             e.synchronize()
             times[i] = min(times[i], s.elapsed_time(e))
     del out
-    # the first candidate (in list order) within TUNE_TOLERANCE of the
-    # fastest: near-ties do not flip the choice from box to box
-    fastest = min(times)
-    best = next(i for i, t in enumerate(times) if t <= fastest * (1 + TUNE_TOLERANCE))
+    best = pick_candidate(times)
     _tune_memo[key] = cands[best]
     try:
         os.makedirs(jit_cache_dir(), exist_ok=True)
@@ -235,6 +232,15 @@ def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
 
 _tune_memo = {}
 TUNE_TOLERANCE = 0.015
+
+
+def pick_candidate(times, tolerance=None):
+    """Index of the first candidate (in list order) whose time is within
+    ``tolerance`` (default TUNE_TOLERANCE) of the fastest: near-ties do not
+    flip the choice from box to box."""
+    tol = TUNE_TOLERANCE if tolerance is None else tolerance
+    fastest = min(times)
+    return next(i for i, t in enumerate(times) if t <= fastest * (1 + tol))
 
 
 def stream_ptr():

@@ -427,3 +427,17 @@ def test_rolling_layouts_with_tangents(pyqc):
     for l in T.clifford_HE_layers(40, 22):
         big.add_layer(l)
     assert not big.plan_options(True).get("flags", 0) & L.PQS_PLAN_ROLLING
+
+
+def test_tuner_pick_prefers_earlier_candidates_within_tolerance():
+    """The tuner takes the first candidate in list order whose time is within
+    its tolerance of the fastest, so near-ties do not flip between boxes, and
+    a clearly faster later candidate still wins."""
+    from pyramaterised_amd import engine as E
+    assert E.pick_candidate([10.0, 9.9, 12.0]) == 0          # 1% slower: a tie, the first wins
+    assert E.pick_candidate([10.0, 9.5, 12.0]) == 1          # 5% faster later candidate wins
+    assert E.pick_candidate([12.0, 10.1, 10.0]) == 1         # first within 1.5% of the fastest
+    assert E.pick_candidate([10.0, 9.9], tolerance=0.0) == 1
+    assert E.pick_candidate([7.0]) == 0
+    # the c128 candidate list starts with the 11-bit rolling shape
+    assert E.tile_candidates(16, 0)[0] == (11, 3, E.L.PQS_PLAN_ROLLING)
