@@ -441,3 +441,13 @@ def test_tuner_pick_prefers_earlier_candidates_within_tolerance():
     assert E.pick_candidate([7.0]) == 0
     # the c128 candidate list starts with the 11-bit rolling shape
     assert E.tile_candidates(16, 0)[0] == (11, 3, E.L.PQS_PLAN_ROLLING)
+
+
+def test_arbgate_is_rejected_by_the_engine(pyqc):
+    """ARBGATE (gates.py:407-435) has no local-gate form: compiling a circuit
+    that holds one raises instead of falling back to a host computation."""
+    qc = pyqc.PQC(3)
+    H = np.diag(np.arange(8.0)).astype(complex)
+    qc.add_layer([pyqc.R_y(0, 3), pyqc.ARBGATE(H)])
+    with pytest.raises(NotImplementedError):
+        qc.ops()
