@@ -523,18 +523,32 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
             }
           }
         };
-        // Phases the plain greedy pass needs for `from`.
+        // The plain greedy pass over `from`, at most `lookahead` phases
+        // ahead: the phases it needs (lookahead + 1 if more), then the ops
+        // still left.  Runs of up to kLongRun ops are rolled out to the end;
+        // longer ones (a whole circuit in one tile has thousands of ops) look
+        // one phase ahead, which keeps planning linear in the run's length
+        // and, measured on 10-16 qubit HEA circuits, also gave fewer phases
+        // than the full roll-out there.
+        struct PhAhead {
+          int phases;
+          size_t left;
+          bool operator<(const PhAhead &b) const { return phases != b.phases ? phases < b.phases : left < b.left; }
+        };
+        constexpr size_t kLongRun = 512;
+        const int lookahead = rem.size() > kLongRun ? 1 : (1 << 30);
         auto greedy_phases = [&](std::vector<int> from) {
           int count = 0;
           qmask R;
           std::vector<int> run, rest;
           while (!from.empty()) {
+            if (count == lookahead) return PhAhead{count + 1, from.size()};
             phase_from(from, 0, R, run, rest);
-            if (run.empty()) return 1 << 30;
+            if (run.empty()) return PhAhead{1 << 30, from.size()};
             from.swap(rest);
             ++count;
           }
-          return count;
+          return PhAhead{count, 0};
         };
         // Window phase search (as for sweeps): also try register sets seeded
         // with kRegBits consecutive tile qubits and keep the phase after
@@ -546,7 +560,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
           phase_from(rem, 0, R, run, rest);
           if (run.empty()) return fail(PQS_ERR_UNSUPPORTED, "phase planner made no progress");
           if (phase_search && !rest.empty()) {
-            int best = greedy_phases(rest);
+            PhAhead best = greedy_phases(rest);
             qmask R2;
             std::vector<int> run2, rest2;
             for (int q0 = 0; q0 + kRegBits <= n; ++q0) {
@@ -554,7 +568,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
               if ((W & tmask) != W) continue;
               phase_from(rem, W, R2, run2, rest2);
               if (run2.empty()) continue;
-              const int c = greedy_phases(rest2);
+              const PhAhead c = greedy_phases(rest2);
               if (c < best) {
                 best = c;
                 R = R2;
