@@ -451,3 +451,21 @@ def test_arbgate_is_rejected_by_the_engine(pyqc):
     qc.add_layer([pyqc.R_y(0, 3), pyqc.ARBGATE(H)])
     with pytest.raises(NotImplementedError):
         qc.ops()
+
+
+def test_planning_cost_stays_bounded(pyqc):
+    """Plan creation stays fast where the searches have the most candidates:
+    a 12-qubit, 100-layer circuit in one tile (a single long register-phase
+    run) and a 24-qubit, 100-layer circuit under rolling layouts (many
+    sweeps)."""
+    import time
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    for n, d, k in ((12, 100, 12), (24, 100, 12)):
+        qc = pyqc.PQC(n)
+        for l in T.clifford_HE_layers(d, n):
+            qc.add_layer(l)
+        t = time.perf_counter()
+        plan = E.Plan(n, qc.ops(), tile_bits=k, low_bits=4, flags=L.PQS_PLAN_ROLLING)
+        assert time.perf_counter() - t < 10.0, (n, d)
+        assert plan.n_sweeps >= 1
