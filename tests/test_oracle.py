@@ -103,6 +103,22 @@ def test_textbook_identities():
         assert np.allclose(OG.ROT[a](0.9), scipy.linalg.expm(-0.45j * OG.PAULI[a]))
 
 
+def test_fsim_matches_generator_form():
+    """fSim(theta, phi) (gates.py:598-606) equals exp(-i theta (XX + YY) / 2)
+    followed by a phase e^{-i phi} on |11>: the exchange block and the sign
+    of both angles are pinned by this independent construction."""
+    import scipy.linalg
+    X, Y = OG.PAULI["x"], OG.PAULI["y"]
+    gen = (np.kron(X, X) + np.kron(Y, Y)) / 2
+    for t, f in ((0.0, 0.0), (0.37, 1.2), (2.9, -0.4), (np.pi / 2, np.pi / 6)):
+        want = np.diag([1, 1, 1, np.exp(-1j * f)]) @ scipy.linalg.expm(-1j * t * gen)
+        assert np.allclose(OG.fsim(t, f), want, atol=1e-14), (t, f)
+        u = OG.fsim(t, f)
+        assert np.allclose(u.conj().T @ u, np.eye(4))
+    # iSWAP-like point: theta = pi/2 moves |01> to -i|10>
+    assert np.allclose(OG.fsim(np.pi / 2, 0.0) @ np.array([0, 1, 0, 0]), [0, 0, -1j, 0])
+
+
 def _random_circuit(n, rng):
     gates = []
     for _ in range(3):
