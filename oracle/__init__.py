@@ -18,7 +18,9 @@ Pinning (each function's docstring names its own pin):
   * brute force: full 2^N operators built with np.kron at tiny N
     (``sim.dense_unitary``) against the einsum simulator;
   * closed forms and invariants: Bell/product-state purities, unitarity,
-    central finite differences of the state for the derivative states.
+    central finite differences of the state for the derivative states, the
+    fSim gate's generator form exp(-i theta (XX + YY) / 2) with the phase on
+    |11>.
 
 Parity unpinned (no reference output and no mathematical fixed point pins
 them): ``gates.sqrt_h`` (element-wise square root applied to a QuTiP Qobj at
