@@ -28,6 +28,8 @@ classes run concurrently on different streams.  A batch's results do not
 depend on what else is in it: every instance runs the same kernels on its
 own state.
 """
+import base64
+import json
 import time
 from collections import OrderedDict, deque
 from concurrent.futures import ThreadPoolExecutor
@@ -74,6 +76,59 @@ class Result:
     @property
     def latency(self):
         return None if self.done is None else self.done - self.arrival
+
+
+# ------------------------------------------------------------ wire format
+# One JSON object per line.  Floats of the parameter vector are written with
+# Python's shortest round-trip repr, and arrays (QFIM, state) as base64 of
+# their little-endian bytes, so a request or result read back is bit-exact.
+
+def _pack_array(a):
+    a = np.ascontiguousarray(a)
+    le = a.astype(a.dtype.newbyteorder("<"), copy=False)
+    return {"dtype": a.dtype.name, "shape": list(a.shape), "data": base64.b64encode(le.tobytes()).decode("ascii")}
+
+
+def _unpack_array(d):
+    dt = np.dtype(d["dtype"]).newbyteorder("<")
+    return np.frombuffer(base64.b64decode(d["data"]), dtype=dt).reshape(d["shape"]).astype(dt.newbyteorder("="))
+
+
+def request_to_json(req):
+    """One request as a JSON line."""
+    return json.dumps({"rid": req.rid, "n": req.n, "depth": req.depth, "kind": req.kind,
+                       "params": [float(x) for x in req.params], "arrival": req.arrival})
+
+
+def request_from_json(line):
+    """A Request from ``request_to_json`` output; the same validation as the
+    constructor (kind, parameter count)."""
+    d = json.loads(line)
+    return Request(d["rid"], d["n"], d["depth"], d["kind"], np.asarray(d["params"], dtype=np.float64),
+                   d.get("arrival", 0.0))
+
+
+def result_to_json(res, with_state=False):
+    """One result as a JSON line: Q for state requests, the QFIM as a packed
+    float64 array, the state (copied to the host) only with ``with_state``,
+    and the server's times."""
+    d = {"rid": res.rid, "kind": res.kind, "n": res.n, "depth": res.depth, "arrival": res.arrival,
+         "launched": res.launched, "done": res.done, "Q": res.Q}
+    if res.qfim is not None:
+        d["qfim"] = _pack_array(np.asarray(res.qfim, dtype=np.float64))
+    if with_state and res.state is not None:
+        st = res.state.detach().cpu().numpy() if isinstance(res.state, torch.Tensor) else np.asarray(res.state)
+        d["state"] = _pack_array(st)
+    return json.dumps(d)
+
+
+def result_from_json(line):
+    """A dict with the fields of ``result_to_json``; arrays as numpy."""
+    d = json.loads(line)
+    for key in ("qfim", "state"):
+        if key in d:
+            d[key] = _unpack_array(d[key])
+    return d
 
 
 def make_workload(n_requests, rate, seed=0, n_range=(6, 20), depth_range=(2, 40), qfim_frac=0.3):

@@ -51,3 +51,34 @@ def test_request_bytes_counts_rolling_jacobian_rows():
         rows = P + 2 + (P if qc.rolling_jacobian() else 0)
         assert SV.Server.request_bytes(stub, req) == 16 * 2 ** n * rows + 48 * P * P
     assert PQC(10).rolling_jacobian() is True
+
+
+def test_wire_format_round_trips_bit_exact():
+    """Requests and results written as JSON lines read back bit-exact
+    (parameters through float repr, arrays through base64 bytes), and a
+    request read back is validated like a constructed one."""
+    import json
+    import torch
+    reqs = SV.make_workload(60, rate=50.0, seed=9)
+    for r in reqs:
+        back = SV.request_from_json(SV.request_to_json(r))
+        assert (back.rid, back.n, back.depth, back.kind, back.arrival) == (r.rid, r.n, r.depth, r.kind, r.arrival)
+        assert back.params.dtype == np.float64 and np.array_equal(back.params.view(np.uint64), r.params.view(np.uint64))
+    bad = json.loads(SV.request_to_json(reqs[0]))
+    bad["params"] = bad["params"][:-1]
+    with pytest.raises(ValueError):
+        SV.request_from_json(json.dumps(bad))
+    rng = np.random.default_rng(4)
+    qreq = SV.Request(7, 3, 2, SV.QFIM, rng.random(12), 1.25)
+    res = SV.Result(qreq)
+    res.qfim = rng.normal(size=(12, 12))
+    res.state = torch.as_tensor(rng.normal(size=8) + 1j * rng.normal(size=8))
+    res.launched, res.done = 1.5, 2.0
+    d = SV.result_from_json(SV.result_to_json(res, with_state=True))
+    assert d["rid"] == 7 and d["kind"] == SV.QFIM and d["Q"] is None and d["done"] == 2.0
+    assert d["qfim"].dtype == np.float64 and np.array_equal(d["qfim"], res.qfim)
+    assert d["state"].dtype == np.complex128 and np.array_equal(d["state"], res.state.numpy())
+    assert "state" not in SV.result_from_json(SV.result_to_json(res))
+    sres = SV.Result(SV.Request(8, 2, 1, SV.STATE_MW, rng.random(4)))
+    sres.Q = 0.123456789012345678
+    assert SV.result_from_json(SV.result_to_json(sres))["Q"] == sres.Q
