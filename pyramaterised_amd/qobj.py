@@ -26,6 +26,10 @@ ATOL = 1e-12  # qutip.settings.atol, used by Qobj.__eq__
 
 
 class _Data:
+    """The ``Qobj.data`` view: QuTiP 4 keeps a CSR matrix there; code that
+    reads it (``state.data.toarray()``, measure.py:335) gets the same
+    accessors over the dense array."""
+
     def __init__(self, arr):
         self._a = arr
 
@@ -34,6 +38,14 @@ class _Data:
 
     def todense(self):
         return np.asmatrix(self._a)
+
+    def tocsr(self):
+        import scipy.sparse
+        return scipy.sparse.csr_matrix(self._a)
+
+    @property
+    def nnz(self):
+        return int(np.count_nonzero(self._a))
 
     @property
     def shape(self):

@@ -59,3 +59,15 @@ def test_magic_conversion_tables_and_reference_formula():
         s = np.sum(np.abs(2 ** (-n / 2) * np.dot(np.conjugate(psi) * sign, psi[add])) ** (2 * alpha))
         val = 1 / (1 - alpha) * np.log(s) - np.log(d)
         assert abs(val - sre_bruteforce(psi, n, alpha)) < 1e-12
+
+
+def test_qobj_data_view_matches_qutip_accessors():
+    """``Qobj.data`` answers the CSR accessors the reference's code uses
+    (``state.data.toarray()``, measure.py:335) with the same numbers."""
+    import pyramaterised_amd as pyqc
+    op = pyqc.Qobj(np.array([[0, 1j], [2.0, 0]]))
+    d = op.data
+    assert np.array_equal(d.toarray(), op.full())
+    assert d.nnz == 2 and d.shape == (2, 2)
+    csr = d.tocsr()
+    assert csr.nnz == 2 and np.array_equal(csr.toarray(), op.full())
