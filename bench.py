@@ -442,9 +442,10 @@ def main():
                    "n_qubits": 16, "layers": 20, "states": head["S"], "pairs": head["S"] // 2,
                    "amplitudes": "complex128"},
         # The headline's sweeps hold 10,000 resident 1 MiB states and run
-        # ~10 register phases per LDS tile: FP64 VALU + LDS bound, so the
-        # compute roofline applies (FP64 vector peak = FP64 matrix peak).
-        "roofline": {"bound": "mfma", "achieved": head["achieved_flops"] / 1e12, "peak": PEAK_FP64 / 1e12,
+        # ~10 register phases per LDS tile: bound by FP64 vector FMAs and LDS
+        # exchange, so the FP64 compute roofline applies (MI355X's FP64
+        # vector peak equals its FP64 matrix peak, 78.6 TFLOP/s).
+        "roofline": {"bound": "fp64-valu", "achieved": head["achieved_flops"] / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": head["achieved_flops"] / PEAK_FP64,
                      "traffic": traffic_per_launch(head),
                      "kernel": "pqs_jit_sweep_* (run-time specialised sweeps, all launches of one evolution)",

@@ -62,7 +62,10 @@ enum pqs_op_kind {
    * ``tangent`` (a no-op for every other instance).  Consecutive PQS_OP_GEN
    * terms with the same tangent form one generator D = sum_t coef_t P_t,
    * P_t = pauli(q0) (x) pauli(q1) (q1 = -1: one qubit; pauli 0=I 1=X 2=Y 3=Z),
-   * coef_t = (m[0], m[1]). */
+   * coef_t = (m[0], m[1]).  A tangent whose terms come in several runs
+   * (separated by other ops: one parameter on several gates) is
+   * differentiated by the product rule: its state is the sum, over the runs,
+   * of the evolution with that run's generator inserted. */
   PQS_OP_GEN = 11,
   /* fSim-family derivative matrices as the reference writes them
    * (gates.py:609-648, 719-737): pauli selects 1 = d_theta, 2 = d_phi,
@@ -171,6 +174,18 @@ int pqs_plan_sweep(const pqs_plan *plan, int32_t s, int32_t *tile_qubits,
 int pqs_plan_sweep_layout(const pqs_plan *plan, int32_t s, int32_t *read_pos, int32_t *write_pos);
 /* First sweep that executes the generator of tangent t (-1 if none). */
 int pqs_plan_tangent_sweep(const pqs_plan *plan, int32_t t, int32_t *sweep);
+/* Number of parts the generator of tangent t was split into.  A merged
+ * generator (a run of PQS_OP_GEN terms of one tangent) that acts
+ * non-diagonally on more qubits than the tile is split into parts of
+ * consecutive terms that each fit a tile, and every further run of a
+ * tangent's terms (see PQS_OP_GEN) is a part too; pqs_jacobian and tangent
+ * runs of pqs_run / pqs_run_batch evolve every part as its own instance
+ * (extra rows from the context workspace) and sum them into the tangent's
+ * state, which is exact by linearity (reference
+ * shared_parameter.derivative, gates.py:451-466, for generators such as
+ * TFIM_layers' shared R_x over all N qubits, templates.py:194-204).  1 for
+ * an unsplit generator in one run. */
+int pqs_plan_tangent_parts(const pqs_plan *plan, int32_t t, int32_t *n_parts);
 
 /* Batched evolution: every instance runs the whole plan on its own state.
  * Replaces PQC.run (circuit.py:118-125), and with tangent >= 0 the

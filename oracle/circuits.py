@@ -335,3 +335,19 @@ def ry_cz_brickwork(p, n):
         gates += [R_y(i, n) for i in range(n)]
         gates += [CZ((i, i + 1), n) for i in range(layer % 2, n - 1, 2)]
     return gates
+
+
+def xxz(p, n, commute=False):
+    """XXZ_layers (templates.py:230-257): per layer four shared blocks --
+    R_zz on the odd links, [R_yy, R_xx] on the odd links (commute as given),
+    R_zz on the even links, [R_yy, R_xx] on the even links.  Even links are
+    (2i-2, 2i-1), odd links (2i-1, 2i mod n), i = 1..n//2."""
+    even = [(2 * i - 2, 2 * i - 1) for i in range(1, n // 2 + 1)]
+    odd = [(2 * i - 1, (2 * i) % n) for i in range(1, n // 2 + 1)]
+    gates = []
+    for _ in range(p):
+        gates.append(shared_parameter([R_zz(pr, n) for pr in odd], n))
+        gates.append(shared_parameter([R_yy(pr, n) for pr in odd] + [R_xx(pr, n) for pr in odd], n, commute))
+        gates.append(shared_parameter([R_zz(pr, n) for pr in even], n))
+        gates.append(shared_parameter([R_yy(pr, n) for pr in even] + [R_xx(pr, n) for pr in even], n, commute))
+    return gates

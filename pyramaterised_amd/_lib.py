@@ -57,6 +57,7 @@ SIGNATURES = {
     "pqs_batch_table": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, i32, vp, vp, vp, vp]),
     "pqs_run_table": (C.c_int, [vp, vp, vp, i64, vp, vp]),
     "pqs_plan_tangent_sweep": (C.c_int, [vp, i32, C.POINTER(i32)]),
+    "pqs_plan_tangent_parts": (C.c_int, [vp, i32, C.POINTER(i32)]),
     "pqs_run": (C.c_int, [vp, vp, C.POINTER(Instance), i64, vp, vp]),
     "pqs_run_batch": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, vp, i32, vp, vp, vp]),
     "pqs_jacobian": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, vp]),

@@ -159,15 +159,46 @@ class PQC:
     # ------------------------------------------------------------ compilation
     def ops(self, with_generators=False):
         """Flat primitive-op list in gate order; with generators, each
-        parameterised gate's derivative op(s) follow its primitives."""
+        parameterised gate's derivative op(s) follow its primitives.  A gate
+        whose derivative is not a generator after it (a commute=False
+        shared_parameter block, gates.py:458-466) gets none here: its
+        tangent comes from ``special_ops``."""
         out = []
         for g in self.gates:
             slot = self._gate_slot.get(id(g), -1)
             out += g.prim_ops(max(slot, 0))
-            if with_generators and g.param_count > 0:
+            if with_generators and g.param_count > 0 and not getattr(g, "special_derivative", False):
                 for w in range(g.param_count):
                     out += g.gen_ops(slot + w, slot, which=w + 1)
         return out
+
+    def special_slots(self):
+        """Parameter slots whose derivative needs a plan of its own."""
+        return [self._gate_slot[id(g)] for g in self.gates
+                if g.param_count > 0 and getattr(g, "special_derivative", False)]
+
+    def special_ops(self, slot):
+        """The circuit with the derivative of the gate at ``slot`` inserted
+        as the reference forms it (take_derivative, circuit.py:149-172, with
+        a commute=False block's derivative, gates.py:458-466): the primitive
+        ops of every gate, and after that gate its ``derivative_ops`` as
+        tangent 0.  Its tangent-0 state is d psi / d theta_slot."""
+        out = []
+        for g in self.gates:
+            s = self._gate_slot.get(id(g), -1)
+            out += g.prim_ops(max(s, 0))
+            if s == slot and g.param_count > 0:
+                out += g.derivative_ops(0, s)
+        return out
+
+    def _special_plan(self, slot, dtype=None):
+        dt = self.dtype if dtype is None else E.dtype_code(dtype)
+        key = ("special", slot, dt)
+        p = self._plans.get(key)
+        if p is None:
+            p = E.Plan(self.n_qubits, self.special_ops(slot), dtype=dt, **self.plan_options(True, dt))
+            self._plans[key] = p
+        return p
 
     # with jit="auto", a plan used this many times gets specialised kernels
     # even for small batches (repeated QFIMs, training loops)
@@ -326,8 +357,15 @@ class PQC:
         """circuit.py:149-172: the circuit with gate g_on replaced by D*gate,
         run on the initial state (one GPU run with the generator inserted)."""
         t = self._tangent_of(g_on, param)
-        out = E.run(self.plan(with_generators=True), self._params_vec(), self.init(), tangents=[t])
+        plan, tg = self._tangent_plan(t)
+        out = E.run(plan, self._params_vec(), self.init(), tangents=[tg])
         return Ket(out[0], self.n_qubits)
+
+    def _tangent_plan(self, t):
+        """(plan, tangent index in it) that yields d psi / d theta_t."""
+        if t in self.special_slots():
+            return self._special_plan(t), 0
+        return self.plan(with_generators=True), t
 
     def get_gradients(self, method="forward"):
         """circuit.py:174-192: every derivative state.  ``forward`` spawns all
@@ -337,16 +375,27 @@ class PQC:
         return [Ket(J[i], self.n_qubits) for i in range(J.shape[0])]
 
     def jacobian(self, method="forward"):
-        """(psi, J) with J[t] = d psi / d theta_t as tensors."""
+        """(psi, J) with J[t] = d psi / d theta_t as tensors.  Rows of
+        commute=False blocks whose derivative is not a generator sum come
+        from a plan of their own each (``special_ops``)."""
         plan = self.plan(with_generators=True, batch=self.n_true_params)
+        specials = self.special_slots()
         if method == "forward":
-            return E.jacobian(plan, self._params_vec(), self.init())
+            J = None
+            if specials:  # the last rows may have no generator in the main plan
+                J = torch.empty((self.n_true_params, 2 ** self.n_qubits), dtype=E.CDTYPE[plan.dtype],
+                                device=E.device())
+            psi, J = E.jacobian(plan, self._params_vec(), self.init(), jac_out=J)
+            for t in specials:
+                E.jacobian(self._special_plan(t), self._params_vec(), self.init(), jac_out=J[t:t + 1])
+            return psi, J
         if method == "plain":
             P = self.n_true_params
             dev = E.device()
             J = torch.empty((P, 2 ** self.n_qubits), dtype=E.CDTYPE[plan.dtype], device=dev)
             for t in range(P):
-                E.run(plan, self._params_vec(), self.init(), tangents=[t], out=J[t:t + 1])
+                tp, tg = self._tangent_plan(t)
+                E.run(tp, self._params_vec(), self.init(), tangents=[tg], out=J[t:t + 1])
             psi = E.run(self.plan(), self._params_vec(), self.init())[0]
             return psi, J
         raise ValueError(f"unknown method {method!r}")

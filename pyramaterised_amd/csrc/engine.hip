@@ -1351,6 +1351,29 @@ __global__ void k_fill_zero(V *p, int64_t n) {
   if (i < n) p[i] = V{0, 0};
 }
 
+// Sums of split-generator parts: row acc[y].dst += every source row
+// srcs[first .. first + count) (in that order; one block row per target).
+struct AccRow {
+  void *dst;
+  int32_t first, count;
+};
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_add_rows(const AccRow *__restrict__ acc, const void *const *__restrict__ srcs,
+                                                  int64_t dim) {
+  const AccRow a = acc[blockIdx.y];
+  V *dst = static_cast<V *>(a.dst);
+  for (int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x; j < dim; j += int64_t(gridDim.x) * 256) {
+    V v = dst[j];
+    for (int c = 0; c < a.count; ++c) {
+      const V w = static_cast<const V *>(srcs[a.first + c])[j];
+      v.x += w.x;
+      v.y += w.y;
+    }
+    dst[j] = v;
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -1417,6 +1440,8 @@ struct pqs_ctx {
   char arch[64] = {0};
   void *ws = nullptr;
   size_t ws_bytes = 0;
+  hipStream_t ws_stream = nullptr;  // stream of the last call that used ws
+  hipEvent_t ws_event = nullptr;
   void *pin = nullptr;
   size_t pin_bytes = 0;
   hipEvent_t pin_event = nullptr;
@@ -1436,22 +1461,34 @@ struct DeviceGuard {
   }
 };
 
-int ws_reserve(pqs_ctx *ctx, size_t bytes) {
-  if (bytes <= ctx->ws_bytes) return PQS_OK;
+// The workspace grows on demand and is allocated and released in stream
+// order (hipMallocAsync / hipFreeAsync), so growing it never waits for the
+// device.  Work on the stream that used it last may still read the old
+// buffer: when that stream is not `st`, `st` first waits for it.
+int ws_reserve(pqs_ctx *ctx, size_t bytes, hipStream_t st) {
+  if (bytes <= ctx->ws_bytes) {
+    ctx->ws_stream = st;
+    return PQS_OK;
+  }
   size_t nb = std::max<size_t>(bytes, size_t(64) << 20);
   nb = (nb + 4095) & ~size_t(4095);
   if (ctx->ws) {
-    HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipFree(ctx->ws));
+    if (ctx->ws_stream != st) {
+      HIPCHECK(hipEventRecord(ctx->ws_event, ctx->ws_stream));
+      HIPCHECK(hipStreamWaitEvent(st, ctx->ws_event, 0));
+    }
+    HIPCHECK(hipFreeAsync(ctx->ws, st));
     ctx->ws = nullptr;
     ctx->ws_bytes = 0;
   }
-  hipError_t e = hipMalloc(&ctx->ws, nb);
+  hipError_t e = hipMallocAsync(&ctx->ws, nb, st);
   if (e != hipSuccess) {
     ctx->ws = nullptr;
+    (void)hipGetLastError();
     return fail(PQS_ERR_NOMEM, "workspace allocation failed");
   }
   ctx->ws_bytes = nb;
+  ctx->ws_stream = st;
   return PQS_OK;
 }
 
@@ -1671,7 +1708,9 @@ int pqs_ctx_create(int device, pqs_ctx **out) {
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
     c->max_lds = v;
   std::snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
-  if (hipEventCreateWithFlags(&c->pin_event, hipEventDisableTiming) != hipSuccess) {
+  if (hipEventCreateWithFlags(&c->pin_event, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ws_event, hipEventDisableTiming) != hipSuccess) {
+    if (c->pin_event) (void)hipEventDestroy(c->pin_event);
     delete c;
     return fail(PQS_ERR_HIP, "hipEventCreate failed");
   }
@@ -1683,9 +1722,13 @@ int pqs_ctx_destroy(pqs_ctx *ctx) {
   if (!ctx) return PQS_OK;
   DeviceGuard g(ctx->device);
   (void)hipDeviceSynchronize();
-  if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->ws) {
+    (void)hipFreeAsync(ctx->ws, ctx->ws_stream);
+    (void)hipStreamSynchronize(ctx->ws_stream);
+  }
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->pin_event) (void)hipEventDestroy(ctx->pin_event);
+  if (ctx->ws_event) (void)hipEventDestroy(ctx->ws_event);
   delete ctx;
   return PQS_OK;
 }
@@ -1712,35 +1755,118 @@ int pqs_plan_destroy(pqs_plan *p) {
   return PQS_OK;
 }
 
-// Rolling-layout plans change the bit layout from sweep to sweep, so every
-// instance runs out of place: its second buffer is taken from the
-// workspace, [alt states | instance table] (returned: the table's offset).
-// An instance whose input is its own output buffer, while the first sweep
-// would write that buffer, first copies the input to its second buffer.
-static int rolling_setup(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, hipStream_t st, size_t *tab_off) {
-  *tab_off = 0;
-  if (!p->rolling) return ws_reserve(ctx, h.size() * sizeof(InstDev));
+// Launch the accumulation of split-generator parts: for every AccRow of the
+// table at d_acc (n_acc rows, sources at d_srcs), dst += sum of its sources.
+static int launch_add_rows(pqs_plan *p, const AccRow *d_acc, const void *const *d_srcs, int64_t n_acc,
+                           hipStream_t st) {
+  if (n_acc <= 0) return PQS_OK;
+  const int64_t dim = int64_t(1) << p->n;
+  const unsigned gx = unsigned(std::min<int64_t>(grid_for(dim, 256), 2048));
+  if (n_acc > 65535) return fail(PQS_ERR_UNSUPPORTED, "too many split-generator rows");
+  if (p->dtype == PQS_C128)
+    hipLaunchKernelGGL(k_add_rows<double2>, dim3(gx, unsigned(n_acc)), dim3(256), 0, st, d_acc, d_srcs, dim);
+  else
+    hipLaunchKernelGGL(k_add_rows<float2>, dim3(gx, unsigned(n_acc)), dim3(256), 0, st, d_acc, d_srcs, dim);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+// Run instances h of plan p (pqs_run, pqs_run_batch).
+//  - An instance whose tangent's generator was split into parts gets one
+//    extra instance per further part, evolving a workspace row from the same
+//    input; after the sweeps the part rows are summed into the instance's
+//    state.  A part that would read the instance's own buffer (in-place
+//    input) starts from a copy of it.
+//  - Rolling-layout plans change the bit layout from sweep to sweep, so every
+//    instance runs out of place: its second buffer is a workspace row.  An
+//    instance whose input is its own output buffer, while the first sweep
+//    would write that buffer, first copies the input to its second buffer.
+// Workspace: [alt rows | part rows | instance table | AccRow table | sources].
+static int run_instances(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, const void *init_tab,
+                         hipStream_t st) {
   const size_t sb = (size_t(1) << p->n) * amp_bytes(p->dtype);
-  const size_t alt_b = h.size() * sb;
-  *tab_off = (alt_b + 255) & ~size_t(255);
-  int rc = ws_reserve(ctx, *tab_off + h.size() * sizeof(InstDev));
+  const size_t n_main = h.size();
+  // extra instances for split generators: (owner, sub-tangent)
+  std::vector<std::pair<size_t, int>> parts;
+  if (p->n_sub > p->n_tangents)
+    for (size_t i = 0; i < n_main; ++i) {
+      const int t = h[i].tangent;
+      if (t < 0) continue;
+      for (int u = p->n_tangents; u < p->n_sub; ++u)
+        if (p->tangent_parent[size_t(u)] == t) parts.push_back({i, u});
+    }
+  const size_t n_all = n_main + parts.size();
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t alt_off = 0;
+  const size_t alt_b = p->rolling ? n_all * sb : 0;
+  const size_t part_off = al(alt_off + alt_b);
+  const size_t tab_off = al(part_off + parts.size() * sb);
+  // one AccRow per owner with parts
+  std::vector<AccRow> acc;
+  std::vector<const void *> srcs;
+  for (size_t k = 0; k < parts.size(); ++k) {
+    void *row_dst = h[parts[k].first].state;
+    if (k == 0 || parts[k - 1].first != parts[k].first)
+      acc.push_back(AccRow{row_dst, int32_t(srcs.size()), 0});
+    srcs.push_back(nullptr);  // filled below (workspace address)
+    ++acc.back().count;
+  }
+  const size_t acc_off = al(tab_off + n_all * sizeof(InstDev));
+  const size_t src_off = al(acc_off + acc.size() * sizeof(AccRow));
+  const size_t total = src_off + srcs.size() * sizeof(void *);
+  int rc = ws_reserve(ctx, total, st);
   if (rc) return rc;
-  const int S = int(p->dev_sweeps.size());
-  const bool first_writes_state = ((S - 1) & 1) == 0;
-  for (size_t i = 0; i < h.size(); ++i) {
-    InstDev &d = h[i];
-    d.alt = (char *)ctx->ws + i * sb;
-    d.pingpong = 1;
-    if (d.init == 0 && (d.src == nullptr || d.src == d.state)) {
-      if (first_writes_state) {
-        HIPCHECK(hipMemcpyAsync(d.alt, d.state, sb, hipMemcpyDeviceToDevice, st));
-        d.src = d.alt;
+  char *ws = (char *)ctx->ws;
+  for (size_t k = 0; k < parts.size(); ++k) {
+    const InstDev &own = h[parts[k].first];
+    InstDev d = own;
+    d.tangent = parts[k].second;
+    d.state = ws + part_off + k * sb;
+    srcs[k] = d.state;
+    if (own.init == 0) {
+      const void *in = own.src ? own.src : own.state;
+      if (in == own.state) {
+        HIPCHECK(hipMemcpyAsync(d.state, own.state, sb, hipMemcpyDeviceToDevice, st));
+        d.src = nullptr;
       } else {
-        d.src = d.state;
+        d.src = in;
+      }
+    }
+    h.push_back(d);
+  }
+  if (p->rolling) {
+    const int S = int(p->dev_sweeps.size());
+    const bool first_writes_state = ((S - 1) & 1) == 0;
+    for (size_t i = 0; i < h.size(); ++i) {
+      InstDev &d = h[i];
+      d.alt = ws + alt_off + i * sb;
+      d.pingpong = 1;
+      if (d.init == 0 && (d.src == nullptr || d.src == d.state)) {
+        if (first_writes_state) {
+          HIPCHECK(hipMemcpyAsync(d.alt, d.state, sb, hipMemcpyDeviceToDevice, st));
+          d.src = d.alt;
+        } else {
+          d.src = d.state;
+        }
       }
     }
   }
-  return PQS_OK;
+  // one upload: instance table, AccRows, source pointers
+  std::vector<char> blob(total - tab_off, 0);
+  std::memcpy(blob.data(), h.data(), h.size() * sizeof(InstDev));
+  if (!acc.empty()) {
+    std::memcpy(blob.data() + (acc_off - tab_off), acc.data(), acc.size() * sizeof(AccRow));
+    std::memcpy(blob.data() + (src_off - tab_off), srcs.data(), srcs.size() * sizeof(void *));
+  }
+  rc = upload(ctx, ws + tab_off, blob.data(), blob.size(), st);
+  if (rc) return rc;
+  const InstDev *d_tab = (const InstDev *)(ws + tab_off);
+  for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
+    rc = launch_sweep(ctx, p, s, d_tab, int64_t(h.size()), init_tab, st);
+    if (rc) return rc;
+  }
+  return launch_add_rows(p, (const AccRow *)(ws + acc_off), (const void *const *)(ws + src_off),
+                         int64_t(acc.size()), st);
 }
 
 int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst, const void *init_tab,
@@ -1770,17 +1896,7 @@ int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst
   }
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
-  size_t tab_off = 0;
-  rc = rolling_setup(ctx, p, h, st, &tab_off);
-  if (rc) return rc;
-  const InstDev *d_tab = (const InstDev *)((char *)ctx->ws + tab_off);
-  rc = upload(ctx, (void *)d_tab, h.data(), h.size() * sizeof(InstDev), st);
-  if (rc) return rc;
-  for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
-    rc = launch_sweep(ctx, p, s, d_tab, n_inst, init_tab, st);
-    if (rc) return rc;
-  }
-  return PQS_OK;
+  return run_instances(ctx, p, h, init_tab, st);
 }
 
 int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride, int64_t batch,
@@ -1811,17 +1927,7 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride,
   }
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
-  size_t tab_off = 0;
-  rc = rolling_setup(ctx, p, h, st, &tab_off);
-  if (rc) return rc;
-  const InstDev *d_tab = (const InstDev *)((char *)ctx->ws + tab_off);
-  rc = upload(ctx, (void *)d_tab, h.data(), h.size() * sizeof(InstDev), st);
-  if (rc) return rc;
-  for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
-    rc = launch_sweep(ctx, p, s, d_tab, batch, init_tab, st);
-    if (rc) return rc;
-  }
-  return PQS_OK;
+  return run_instances(ctx, p, h, init_tab, st);
 }
 
 int64_t pqs_batch_table_bytes(int64_t batch) { return batch < 0 ? 0 : batch * int64_t(sizeof(InstDev)); }
@@ -1887,24 +1993,55 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
   if (rc) return rc;
   // Rolling layouts write every sweep out of place, so each tangent row
   // ping-pongs with a workspace row as the primal does; fixed layouts update
-  // the rows in place.  Workspace: [alt state | alt rows | instance table]
+  // the rows in place.  The parts of split generators (sub-tangents
+  // n_tangents..n_sub-1) evolve workspace rows that are summed into their
+  // tangent's row at the end.  Workspace: [alt state | alt rows | part rows |
+  // instance table | AccRow table | sources]
   const bool rows_pp = p->rolling && S > 1;
   if (rows_pp && init == 0 && src == psi)
     return fail(PQS_ERR_ARG, "with a rolling-layout plan the source state must not be the psi buffer");
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   const size_t sb = size_t(dim) * ab;
   const size_t alt_b = (S > 1) ? sb : 0;
-  const size_t rows_off = (alt_b + 255) & ~size_t(255);
-  const size_t rows_b = rows_pp ? size_t(p->n_tangents) * sb : 0;
-  const size_t tab_off = (rows_off + rows_b + 255) & ~size_t(255);
-  const size_t tb = size_t(1 + p->n_tangents) * sizeof(InstDev);
-  rc = ws_reserve(ctx, tab_off + tb);
+  const size_t rows_off = al(alt_b);
+  const size_t rows_b = rows_pp ? size_t(p->n_sub) * sb : 0;
+  const int n_parts = p->n_sub - p->n_tangents;
+  const size_t part_off = al(rows_off + rows_b);
+  const size_t tab_off = al(part_off + size_t(n_parts) * sb);
+  const size_t tb = size_t(1 + p->n_sub) * sizeof(InstDev);
+  // AccRows: one per tangent with parts
+  std::vector<AccRow> acc;
+  std::vector<const void *> srcs;
+  for (int t = 0; t < p->n_tangents && n_parts > 0; ++t) {
+    AccRow a{(char *)jac + size_t(t) * sb, int32_t(srcs.size()), 0};
+    for (int u = p->n_tangents; u < p->n_sub; ++u)
+      if (p->tangent_parent[size_t(u)] == t) {
+        srcs.push_back(nullptr);
+        ++a.count;
+      }
+    if (a.count) acc.push_back(a);
+  }
+  const size_t acc_off = al(tab_off + tb);
+  const size_t src_off = al(acc_off + acc.size() * sizeof(AccRow));
+  const size_t total = src_off + srcs.size() * sizeof(void *);
+  rc = ws_reserve(ctx, total, st);
   if (rc) return rc;
   char *ws = (char *)ctx->ws;
   void *alt = S > 1 ? (void *)ws : nullptr;
   auto W = [&](int s) -> void * { return ((S - 1 - s) & 1) == 0 ? psi : alt; };
+  auto row_of = [&](int t) -> void * {
+    return t < p->n_tangents ? (void *)((char *)jac + size_t(t) * sb)
+                             : (void *)(ws + part_off + size_t(t - p->n_tangents) * sb);
+  };
+  {
+    size_t k = 0;
+    for (int t = 0; t < p->n_tangents && n_parts > 0; ++t)
+      for (int u = p->n_tangents; u < p->n_sub; ++u)
+        if (p->tangent_parent[size_t(u)] == t) srcs[k++] = row_of(u);
+  }
 
   std::vector<int> order;
-  for (int t = 0; t < p->n_tangents; ++t)
+  for (int t = 0; t < p->n_sub; ++t)
     if (p->tangent_sweep[t] >= 0) order.push_back(t);
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return p->tangent_sweep[a] < p->tangent_sweep[b]; });
@@ -1922,7 +2059,7 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
   for (int t : order) {
     InstDev d{};
     const int s0 = p->tangent_sweep[t];
-    d.state = (char *)jac + size_t(t) * size_t(dim) * ab;
+    d.state = row_of(t);
     d.params = params;
     d.tangent = t;
     d.start_sweep = s0;
@@ -1947,8 +2084,14 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
       hipLaunchKernelGGL(k_fill_zero<float2>, dim3(grid_for(dim, 256)), dim3(256), 0, st, (float2 *)row, dim);
     HIPCHECK(hipGetLastError());
   }
+  std::vector<char> blob(total - tab_off, 0);
+  std::memcpy(blob.data(), h.data(), h.size() * sizeof(InstDev));
+  if (!acc.empty()) {
+    std::memcpy(blob.data() + (acc_off - tab_off), acc.data(), acc.size() * sizeof(AccRow));
+    std::memcpy(blob.data() + (src_off - tab_off), srcs.data(), srcs.size() * sizeof(void *));
+  }
   InstDev *d_tab = (InstDev *)(ws + tab_off);
-  rc = upload(ctx, d_tab, h.data(), h.size() * sizeof(InstDev), st);
+  rc = upload(ctx, d_tab, blob.data(), blob.size(), st);
   if (rc) return rc;
   size_t active = 1;
   for (int s = 0; s < S; ++s) {
@@ -1956,7 +2099,8 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
     rc = launch_sweep(ctx, p, s, d_tab, int64_t(active), init_tab, st);
     if (rc) return rc;
   }
-  return PQS_OK;
+  return launch_add_rows(p, (const AccRow *)(ws + acc_off), (const void *const *)(ws + src_off),
+                         int64_t(acc.size()), st);
 }
 
 int pqs_inner(pqs_ctx *ctx, int32_t dtype, const void *base, int64_t dim, const int64_t *a_off,
@@ -1967,7 +2111,7 @@ int pqs_inner(pqs_ctx *ctx, int32_t dtype, const void *base, int64_t dim, const 
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
   const int64_t nch = (dim + kInnerChunk - 1) / kInnerChunk;
-  int rc = ws_reserve(ctx, size_t(n_pairs * nch) * sizeof(double2));
+  int rc = ws_reserve(ctx, size_t(n_pairs * nch) * sizeof(double2), st);
   if (rc) return rc;
   double2 *part = (double2 *)ctx->ws;
   const int64_t grid = n_pairs * nch;
@@ -1993,7 +2137,7 @@ int pqs_mw(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t b
   hipStream_t st = (hipStream_t)stream;
   const int c = std::min(n, 11);
   const int64_t nch = int64_t(1) << (n - c);
-  int rc = ws_reserve(ctx, size_t(batch * nch * n * 4) * sizeof(double));
+  int rc = ws_reserve(ctx, size_t(batch * nch * n * 4) * sizeof(double), st);
   if (rc) return rc;
   double *part = (double *)ctx->ws;
   const int64_t grid = batch * nch;
@@ -2085,10 +2229,10 @@ int pqs_gram(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m,
   if (!ctx || !a || !b || !out) return fail(PQS_ERR_ARG, "NULL argument");
   if (m <= 0 || nn <= 0 || dim <= 0 || lda < dim || ldb < dim) return fail(PQS_ERR_ARG, "bad sizes");
   DeviceGuard g(ctx->device);
-  int rc = ws_reserve(ctx, gram_part_bytes(ctx, m, nn, dim));
+  hipStream_t st = (hipStream_t)stream;
+  int rc = ws_reserve(ctx, gram_part_bytes(ctx, m, nn, dim), st);
   if (rc) return rc;
-  return gram_impl(ctx, dtype, a, lda, m, b, ldb, nn, dim, (double2 *)out, (double2 *)ctx->ws,
-                   (hipStream_t)stream);
+  return gram_impl(ctx, dtype, a, lda, m, b, ldb, nn, dim, (double2 *)out, (double2 *)ctx->ws, st);
 }
 
 int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void *psi, int64_t dim, double *F,
@@ -2117,7 +2261,7 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
     const size_t off_c = (part_b + 255) & ~size_t(255);
     const size_t off_b = (off_c + size_t(P * P) * sizeof(double) + 255) & ~size_t(255);
     const size_t off_t = (off_b + size_t(P) * sizeof(double2) + 255) & ~size_t(255);
-    int rc = ws_reserve(ctx, off_t + tl.size() * sizeof(int2));
+    int rc = ws_reserve(ctx, off_t + tl.size() * sizeof(int2), st);
     if (rc) return rc;
     char *ws = (char *)ctx->ws;
     double *C = (double *)(ws + off_c);
@@ -2142,7 +2286,7 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
   const size_t part_b = std::max(gram_part_bytes(ctx, P, P, dim), gram_part_bytes(ctx, 1, P, dim));
   const size_t off_g = (part_b + 255) & ~size_t(255);
   const size_t off_b = (off_g + size_t(P * P) * sizeof(double2) + 255) & ~size_t(255);
-  int rc = ws_reserve(ctx, off_b + size_t(P) * sizeof(double2));
+  int rc = ws_reserve(ctx, off_b + size_t(P) * sizeof(double2), st);
   if (rc) return rc;
   char *ws = (char *)ctx->ws;
   double2 *part = (double2 *)ws;
@@ -2187,7 +2331,7 @@ int pqs_expect_pauli(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n,
   }
   const size_t part_b = size_t(batch * nch) * sizeof(double2);
   const size_t off_t = (part_b + 255) & ~size_t(255);
-  int rc = ws_reserve(ctx, off_t + h.size() * sizeof(PauliTerm));
+  int rc = ws_reserve(ctx, off_t + h.size() * sizeof(PauliTerm), st);
   if (rc) return rc;
   char *ws = (char *)ctx->ws;
   PauliTerm *d_terms = (PauliTerm *)(ws + off_t);
@@ -2243,7 +2387,7 @@ int pqs_magic(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
   const int64_t D = int64_t(1) << n;
-  int rc = ws_reserve(ctx, size_t(batch * D) * sizeof(double));
+  int rc = ws_reserve(ctx, size_t(batch * D) * sizeof(double), st);
   if (rc) return rc;
   const size_t lds = size_t(D) * sizeof(double2);
   if (int(lds) > ctx->max_lds) return fail(PQS_ERR_UNSUPPORTED, "state does not fit in LDS");

@@ -138,6 +138,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
   p.exec.clear();
   p.sweeps.clear();
   p.n_tangents = 0;
+  p.n_sub = 0;
 
   for (int i = 0; i < n_ops; ++i) {
     int rc = validate(n, ops[i], i);
@@ -145,10 +146,35 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     if (ops[i].tangent >= 0) p.n_tangents = std::max(p.n_tangents, ops[i].tangent + 1);
   }
 
-  // Merge consecutive generator terms of one tangent into a Pauli sum.
+  // Merge consecutive generator terms of one tangent into a Pauli sum.  A sum
+  // that acts non-diagonally on more qubits than a tile holds is split into
+  // parts of consecutive terms, each within the tile; the first part keeps
+  // the tangent, every further part becomes a sub-tangent whose state is
+  // summed into the tangent's row after the run.  D psi = sum_parts D_part
+  // psi, so the split is exact (reference shared_parameter.derivative with
+  // commute=True sums the sub-gates' generators, gates.py:451-457).
+  //
+  // A tangent whose generator terms come in several runs (separated by other
+  // ops: one parameter on several gates) is differentiated by the product
+  // rule: every run after its first is a sub-tangent too, so the tangent's
+  // state is the sum over the runs of the evolution with that run's
+  // generator inserted.
+  p.n_sub = p.n_tangents;
+  p.tangent_parent.clear();
+  for (int t = 0; t < p.n_tangents; ++t) p.tangent_parent.push_back(t);
+  std::vector<char> gen_seen(size_t(p.n_tangents), 0);
+  auto part_tangent = [&](int t, bool first_part) {
+    if (first_part && !gen_seen[size_t(t)]) {
+      gen_seen[size_t(t)] = 1;
+      return t;
+    }
+    p.tangent_parent.push_back(t);
+    return p.n_sub++;
+  };
   for (int i = 0; i < n_ops;) {
     ExecOp e;
     e.first = i;
+    e.tangent = ops[i].tangent;
     masks_of(n, ops[i], e.active, e.diag);
     if (ops[i].kind == PQS_OP_GEN) {
       int j = i + 1;
@@ -162,9 +188,49 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
       e.diag &= ~e.active;
       e.kind = kPauliSum;
       e.n_src = j - i;
+      if (popc(e.active) <= k) e.tangent = part_tangent(ops[i].tangent, true);
+      if (popc(e.active) > k) {
+        // parts of at most `cap` non-diagonal qubits, balanced over the
+        // fewest parts that a cap of k - 2 allows (room left in the tile)
+        const int wide = popc(e.active);
+        const int cap0 = std::max(2, k - 2);
+        const int n_parts = (wide + cap0 - 1) / cap0;
+        const int cap = std::max(2, std::min(k, (wide + n_parts - 1) / n_parts));
+        int first = i;
+        bool first_part = true;
+        while (first < j) {
+          ExecOp part;
+          part.kind = kPauliSum;
+          part.first = first;
+          int last = first;
+          while (last < j) {
+            qmask a, d;
+            masks_of(n, ops[last], a, d);
+            if (last > first && popc(part.active | a) > cap) break;
+            part.active |= a;
+            part.diag |= d;
+            ++last;
+          }
+          part.diag &= ~part.active;
+          part.n_src = last - first;
+          if (popc(part.active) > k) {
+            char buf[200];
+            std::snprintf(buf, sizeof buf, "generator term %d acts non-diagonally on more than the %d-qubit tile",
+                          first, k);
+            return fail(PQS_ERR_UNSUPPORTED, buf);
+          }
+          part.tangent = part_tangent(ops[i].tangent, first_part);
+          first_part = false;
+          p.exec.push_back(part);
+          first = last;
+        }
+        i = j;
+        continue;
+      }
       i = j;
     } else {
       e.kind = ops[i].kind;
+      if (e.kind == PQS_OP_GEN_FSIM) e.tangent = part_tangent(ops[i].tangent, true);
       ++i;
     }
     if (popc(e.active) > k) {
@@ -403,11 +469,11 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
     }
   }
 
-  p.tangent_sweep.assign(p.n_tangents, -1);
+  p.tangent_sweep.assign(p.n_sub, -1);
   for (size_t s = 0; s < p.sweeps.size(); ++s)
     for (int ei : p.sweeps[s].ops) {
-      const pqs_op &so = ops[p.exec[ei].first];
-      if (so.tangent >= 0 && p.tangent_sweep[so.tangent] < 0) p.tangent_sweep[so.tangent] = int(s);
+      const int tg = p.exec[ei].tangent;
+      if (tg >= 0 && p.tangent_sweep[tg] < 0) p.tangent_sweep[tg] = int(s);
     }
 
   // ---- register phases and device image
@@ -789,7 +855,7 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         const pqs_op &so = ops[e.first];
         DevOp d{};
         d.kind = e.kind == kPauliSum ? ((p.reg_exec && !ph.lds) ? kGen1 : kPauliSum) : so.kind;
-        d.tangent = so.tangent;
+        d.tangent = e.tangent;
         d.slot = so.slot;
         d.slot2 = so.slot2;
         d.pauli = so.pauli;
@@ -925,6 +991,16 @@ int pqs_plan_sweep(const pqs_plan *p, int32_t s, int32_t *tile_qubits, int32_t *
   if (n_op) *n_op = int32_t(sw.ops.size());
   if (op_index)
     for (size_t i = 0; i < sw.ops.size(); ++i) op_index[i] = p->exec[sw.ops[i]].first;
+  return PQS_OK;
+}
+
+int pqs_plan_tangent_parts(const pqs_plan *p, int32_t t, int32_t *n_parts) {
+  if (!p || !n_parts) return pqs::fail(PQS_ERR_ARG, "NULL argument");
+  if (t < 0 || t >= p->n_tangents) return pqs::fail(PQS_ERR_ARG, "tangent out of range");
+  int c = 0;
+  for (int u = 0; u < p->n_sub; ++u)
+    if (p->tangent_parent[size_t(u)] == t && (u == t || u >= p->n_tangents)) ++c;
+  *n_parts = c;
   return PQS_OK;
 }
 

@@ -23,6 +23,7 @@ struct ExecOp {
   int n_src = 1;         // number of source ops merged (generator terms)
   qmask active = 0;      // qubits the op acts on non-diagonally
   qmask diag = 0;        // qubits the op acts on diagonally (block-diagonal)
+  int tangent = -1;      // derivative ops: tangent (or sub-tangent) index
 };
 constexpr int kPauliSum = 100;
 // A one-term, one-qubit generator (coef * Pauli on q0, applied only by the
@@ -107,11 +108,16 @@ struct pqs_plan {
   int dtype = 0;
   int flags = 0;
   bool rolling = false;  // rolling layouts in effect (PQS_PLAN_ROLLING honoured)
-  int n_tangents = 0;
+  int n_tangents = 0;     // tangents of the caller's op list
+  // A generator wider than the tile is split into parts; every part after
+  // the first runs as a sub-tangent n_tangents.. whose state is summed into
+  // its parent's row at the end of a run.  n_sub counts all of them.
+  int n_sub = 0;
+  std::vector<int> tangent_parent;  // per tangent / sub-tangent: caller tangent
   std::vector<pqs_op> src_ops;
   std::vector<pqs::ExecOp> exec;
   std::vector<pqs::Sweep> sweeps;
-  std::vector<int> tangent_sweep;  // first sweep holding tangent t's generator
+  std::vector<int> tangent_sweep;  // first sweep holding (sub-)tangent t's generator
 
   // host copy of the device image
   std::vector<pqs::DevOp> dev_ops;

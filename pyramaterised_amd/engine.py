@@ -344,6 +344,12 @@ class Plan:
         check(L.lib().pqs_plan_tangent_sweep(self.handle, t, C.byref(s)))
         return s.value
 
+    def tangent_parts(self, t):
+        """Parts the generator of tangent t was split into (1: unsplit)."""
+        n = C.c_int32()
+        check(L.lib().pqs_plan_tangent_parts(self.handle, t, C.byref(n)))
+        return n.value
+
 
 # ------------------------------------------------------------------- init state
 

@@ -565,10 +565,49 @@ class ARBGATE(Gate):
 
 # ------------------------------------------------------------ shared parameter
 
+def _conj_op(o):
+    """The element-wise complex conjugate of a primitive op, as an op of the
+    same kind (rotations: angle -> -angle; R_y, CNOT and CZ are real)."""
+    c = _op(o.kind, o.q0, o.q1, slot=o.slot, slot2=o.slot2, scale=o.scale, offset=o.offset,
+            tangent=o.tangent, pauli=o.pauli)
+    for i in range(32):
+        c.m[i] = o.m[i]
+    if o.kind in (L.OP_MAT1, L.OP_MAT2):
+        for i in range(1, 32, 2):
+            c.m[i] = -o.m[i]
+    elif o.kind in (L.OP_RX, L.OP_RZ, L.OP_RXX, L.OP_RYY, L.OP_RZZ):
+        c.scale, c.offset = -o.scale, -o.offset
+    elif o.kind == L.OP_FSIM and o.slot2 < 0:
+        c.scale, c.offset = -o.scale, -o.offset  # fixed_fSim: phi = 0
+    elif o.kind not in (L.OP_RY, L.OP_CX, L.OP_CZ):
+        raise NotImplementedError(f"no conjugate form for op kind {o.kind}")
+    return c
+
+
+def _pauli_string(g):
+    """{qubit: 'x'|'y'|'z'} of a Pauli rotation gate, or None for any other
+    gate (fixed gates, I, fSim, blocks)."""
+    if isinstance(g, (_Fixed1, I, shared_parameter)) or not isinstance(g, PRot):
+        return None
+    if isinstance(g, RR):
+        return {g.q1: g.axis, g.q2: g.axis}
+    if g.axis in ("x", "y", "z") and type(g).prim_ops is PRot.prim_ops:
+        return {g.q_on: g.axis}
+    return None
+
+
 class shared_parameter(PRot):
     """gates.py:441-484: one angle on every sub-gate, U = prod(layer[::-1])
     (first listed applied first).  With commute=True the derivative is the
-    sum of the sub-gates' generators (gates.py:451-457)."""
+    sum of the sub-gates' generators (gates.py:451-457).  With commute=False
+    the reference's derivative is sum_k prod(layer with g_k -> D_k g_k)
+    times conj(U) (gates.py:458-466), applied after U by take_derivative;
+    where every sub-gate is a Pauli rotation whose string is symmetric (an
+    even number of Y) and the strings commute pairwise, conj(U) = U^dag and
+    each D_k commutes with the block, so that form is the generator sum too.
+    Other commute=False blocks are differentiated through a plan of their own
+    (``PQC`` builds it: the circuit with conj(U) and the interleaved
+    D_k inserted)."""
 
     def __init__(self, layer, q_N, commute=True):
         self.layer = layer
@@ -582,12 +621,69 @@ class shared_parameter(PRot):
     def prim_ops(self, slot):
         return [o for g in self.layer for o in g.prim_ops(slot)]
 
+    def generator_sum(self):
+        """Whether the derivative is the sum of the sub-gates' generators
+        inserted after the block (always for commute=True)."""
+        if self.commute:
+            return True
+        strings = [_pauli_string(g) for g in self.layer]
+        if any(s is None for s in strings):
+            return False
+        for s in strings:
+            if sum(1 for a in s.values() if a == "y") % 2:
+                return False  # P^T = -P: U is not symmetric
+        for i in range(len(strings)):
+            for j in range(i + 1, len(strings)):
+                anti = sum(1 for q, a in strings[i].items() if q in strings[j] and strings[j][q] != a)
+                if anti % 2:
+                    return False
+        return True
+
+    @property
+    def special_derivative(self):
+        return not self.generator_sum()
+
     def gen_ops(self, tangent, slot, which=1):
-        if not self.commute:
+        if not self.generator_sum():
             raise NotImplementedError(
-                "derivative of a non-commuting shared_parameter block (gates.py:458-466) "
-                "is not supported by the GPU engine")
+                "the commute=False derivative of this block (gates.py:458-466) needs the circuit around it: "
+                "use derivative_ops(), which PQC uses through a plan of its own")
         return [o for g in self.layer for o in g.gen_ops(tangent, slot)]
+
+    def derivative_ops(self, tangent, slot):
+        """Ops that map the state after the block, U psi, to the reference's
+        derivative state of the block (gates.py:458-466 followed by the
+        multiplication by U in take_derivative, circuit.py:165-169):
+        conj(U) (the conjugated sub-gates in order), then the sub-gates again
+        with D_k (tangent ``tangent``) right after g_k.  Each D_k is its own
+        run of generator terms, so the engine sums the k terms (product
+        rule over runs of one tangent)."""
+        out = [_conj_op(o) for g in self.layer for o in g.prim_ops(slot)]
+        for g in self.layer:
+            out += g.prim_ops(slot)
+            out += g.gen_ops(tangent, slot)
+        return out
+
+    def derivative(self):
+        if self.generator_sum():
+            return LocalOperator(self.q_N, self.gen_ops(0, 0))
+        # D with D * U = the derivative: the terms act after U (the caller
+        # multiplies D by the gate), with the angle bound to self.theta
+        th = float(self.theta)
+        terms = []
+        conj = [_conj_op(o) for g in self.layer for o in g._const_ops(th, 0.0)]
+        for k in range(len(self.layer)):
+            ops = list(conj)
+            for j, g in enumerate(self.layer):
+                ops += g._const_ops(th, 0.0)
+                if j == k:
+                    for o in g.gen_ops(0, 0):
+                        if o.slot >= 0:  # fSim-family derivative: bind its angle
+                            o.offset = o.scale * th + o.offset
+                            o.scale, o.slot = 0.0, -1
+                        ops.append(o)
+            terms.append(LocalOperator(self.q_N, ops))
+        return LocalOperator(self.q_N, [], terms=terms)
 
     def set_theta(self, theta):
         self.theta = theta

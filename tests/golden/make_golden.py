@@ -107,23 +107,27 @@ def config4():
 
 def mw():
     bell = np.zeros(4, dtype=np.complex128)
-    bell[1] = bell[2] = 1 / np.sqrt(2)  # bell_state('11') = (|01> + |10>)/sqrt(2)
+    bell[1], bell[2] = 1 / np.sqrt(2), -1 / np.sqrt(2)  # QuTiP bell_state('11') = (|01> - |10>)/sqrt(2)
     rng = np.random.default_rng(9)
     singles = rng.normal(size=(5, 2)) + 1j * rng.normal(size=(5, 2))
     singles /= np.linalg.norm(singles, axis=1, keepdims=True)
     prod = singles[0]
     for s in singles[1:]:
         prod = np.kron(prod, s)
-    return dict(bell_Q=OM.single_Q(bell, 2), product_singles=singles, product_Q=OM.single_Q(prod, 5))
+    return dict(bell_state=bell, bell_Q=OM.single_Q(bell, 2), product_singles=singles,
+                product_Q=OM.single_Q(prod, 5))
 
 
-def main():
+def main(names=None):
     for name, fn in (("qg", qg), ("config1", config1), ("npqc", npqc), ("config3", config3),
                      ("config4_22q", config4), ("mw", mw)):
+        if names and name not in names:
+            continue
         d = fn()
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **{k: np.asarray(v) for k, v in d.items()})
         print("wrote", name, flush=True)
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    main(sys.argv[1:])

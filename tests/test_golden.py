@@ -38,6 +38,18 @@ def test_mw_fixture_closed_forms():
     g = load("mw")
     assert abs(float(g["bell_Q"]) - 1) < 1e-12  # tests.py:284-295
     assert abs(float(g["product_Q"])) < 1e-12
+    # QuTiP's bell_state('11') is the singlet (|01> - |10>)/sqrt(2): antisymmetric
+    # under the swap of the two qubits, unit norm
+    b = g["bell_state"]
+    assert np.allclose(b, np.array([0, 1, -1, 0]) / np.sqrt(2), atol=1e-15)
+    swap = b.reshape(2, 2).T.reshape(-1)
+    assert np.allclose(swap, -b, atol=1e-15)
+
+
+def test_package_bell_state_matches_fixture():
+    from pyramaterised_amd import qobj
+    g = load("mw")
+    assert np.max(np.abs(qobj.bell_state("11").full().ravel() - g["bell_state"])) < 1e-15
 
 
 def test_config1_and_config3_fixtures_reproduce():

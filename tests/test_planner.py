@@ -115,11 +115,125 @@ def test_plan_rejects_bad_ops(pyqc):
         E.Plan(3, [_op(L.OP_CX, 1, 1)], tile_bits=3)
     with pytest.raises(L.EngineError):
         E.Plan(3, [_op(L.OP_RX, 0, slot=0)], tile_bits=14)
-    # a generator acting on more qubits than the tile cannot be scheduled
+    # a single generator term wider than the tile cannot be scheduled
     from pyramaterised_amd.gates import _gen
-    ops = [_gen(0, 1, [q], -0.5j) for q in range(6)]
     with pytest.raises(L.EngineError):
-        E.Plan(6, ops, tile_bits=4, low_bits=2)
+        E.Plan(3, [_gen(0, 1, [0, 1], -0.5j)], tile_bits=1, low_bits=1)
+
+
+def _gen_parts(plan, ops):
+    """Per tangent: the generator terms of each part, from the sweeps' op
+    indices (a merged generator reports the index of its first term)."""
+    reported = {i for _, idx in plan.sweeps() for i in idx}
+    parts = {}
+    i = 0
+    while i < len(ops):
+        if ops[i].kind != L.OP_GEN:
+            i += 1
+            continue
+        j = i
+        while j < len(ops) and ops[j].kind == L.OP_GEN and ops[j].tangent == ops[i].tangent:
+            j += 1
+        starts = [k for k in range(i, j) if k in reported]
+        assert starts and starts[0] == i
+        parts.setdefault(ops[i].tangent, []).extend(list(range(a, b)) for a, b in zip(starts, starts[1:] + [j]))
+        i = j
+    return parts
+
+
+def _gen_term(o):
+    code = o.pauli
+    P = [OG.I2, OG.SX, OG.SY, OG.SZ][code]
+    qs = (o.q0,) if o.q1 < 0 else (o.q0, o.q1)
+    m = P if len(qs) == 1 else np.kron(P, P)
+    return qs, complex(o.m[0], o.m[1]) * m
+
+
+def _derivative_by_parts(plan, ops, n, params, t, parts, init=None):
+    """d psi / d theta_t as the engine forms it: every part of tangent t's
+    generator evolved through the sweep order on its own, then summed."""
+    order = [i for _, idx in plan.sweeps() for i in idx]
+    gen_first = {part[0]: part for ps in parts.values() for part in ps}
+    total = np.zeros(2 ** n, dtype=complex)
+    for part in parts[t]:
+        psi = OS.zero_state(n) if init is None else init.copy()
+        for i in order:
+            o = ops[i]
+            if o.kind == L.OP_GEN:
+                if i != part[0]:
+                    continue
+                acc = np.zeros_like(psi)
+                for k in gen_first[i]:
+                    qs, m = _gen_term(ops[k])
+                    acc += OS.apply(psi, qs, m, n)
+                psi = acc
+                continue
+            qs, m = op_matrix(o, params)
+            psi = OS.apply(psi, qs, m, n)
+        total += psi
+    return total
+
+
+@pytest.mark.parametrize("rolling", [False, True])
+def test_wide_generators_split_into_parts_match_oracle(pyqc, rolling):
+    """A shared-parameter generator wider than the tile (TFIM_layers' shared
+    R_x over all N qubits, templates.py:194-204; reference derivative
+    gates.py:451-457) is split into parts that each fit the tile; summing the
+    parts' derivative states, each evolved through the plan's sweep order,
+    gives the oracle's derivative states."""
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    n, p = 9, 2
+    qc = pyqc.PQC(n)
+    for l in T.TFIM_layers(p, n):
+        qc.add_layer(l)
+    ops = qc.ops(True)
+    flags = L.PQS_PLAN_ROLLING if rolling else 0
+    plan = E.Plan(n, ops, tile_bits=6, low_bits=2, flags=flags)
+    parts = _gen_parts(plan, ops)
+    for t in range(qc.n_true_params):
+        want = 3 if t % 2 == 1 else 1  # R_x generators: 9 qubits in parts of <= k - 2 = 4; R_zz sums are diagonal
+        assert plan.tangent_parts(t) == len(parts[t]) == want, t
+    rng = np.random.default_rng(3)
+    th = rng.random(qc.n_true_params) * 2 * np.pi
+    og = OC.tfim(p, n)
+    ref = OS.derivative_states(og, th, n)
+    for t in range(qc.n_true_params):
+        d = _derivative_by_parts(plan, ops, n, th, t, parts)
+        assert np.max(np.abs(d - ref[t])) < 1e-12, t
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_tfim_derivative_plans_at_13_to_24_qubits(pyqc, dtype):
+    """Derivative plans of the reference's TFIM templates plan at every width
+    from 13 to 24 qubits with the derivative tile shapes (12-bit c128,
+    13-bit c64): the shared R_x generators split into parts within the tile,
+    every part's terms sit in the sweep that runs it, and the R_zz sums
+    (diagonal) stay whole."""
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    for n in range(13, 25):
+        for layers in (T.TFIM_layers(2, n), T.tfim_open_layers(2, n), T.modified_TFIM_layers(1, n)):
+            qc = pyqc.PQC(n, dtype=dtype)
+            for l in layers:
+                qc.add_layer(l)
+            ops = qc.ops(True)
+            plan = E.Plan(n, ops, dtype=dtype, **qc.plan_options(True, dtype))
+            k = plan.tile_bits
+            parts = _gen_parts(plan, ops)
+            sweeps = plan.sweeps()
+            for t in range(qc.n_true_params):
+                np_t = plan.tangent_parts(t)
+                assert np_t == len(parts[t]) >= 1
+                for part in parts[t]:
+                    act = set()
+                    for i in part:
+                        act |= active_qubits(ops[i])
+                    assert len(act) <= k
+                    holders = [tq for tq, idx in sweeps if part[0] in idx]
+                    assert len(holders) == 1 and act <= set(holders[0])
+            wide = [t for t in range(qc.n_true_params) if plan.tangent_parts(t) > 1]
+            assert bool(wide) == (n > k), (n, k)
 
 
 @pytest.mark.parametrize("n,tile,low,seed", [(6, 3, 2, 0), (8, 5, 3, 1), (10, 4, 1, 2), (12, 6, 3, 3),
@@ -469,3 +583,59 @@ def test_planning_cost_stays_bounded(pyqc):
         plan = E.Plan(n, qc.ops(), tile_bits=k, low_bits=4, flags=L.PQS_PLAN_ROLLING)
         assert time.perf_counter() - t < 10.0, (n, d)
         assert plan.n_sweeps >= 1
+
+
+def _xxz_pair(pyqc, n):
+    from pyramaterised_amd import templates as T
+    qc = pyqc.PQC(n)
+    for l in T.XXZ_layers(2, n):
+        qc.add_layer(l)
+    return qc, OC.xxz(2, n)
+
+
+def _noncommuting(pyqc, n):
+    """A hardware-efficient layer, a genuinely non-commuting commute=False
+    block [R_x(0), R_y(0), R_zz(0, 1)], then more gates."""
+    pk = [pyqc.R_y(q, n) for q in range(n)] + [pyqc.CNOT([q, q + 1], n) for q in range(n - 1)]
+    pk += [pyqc.shared_parameter([pyqc.R_x(0, n), pyqc.R_y(0, n), pyqc.R_zz([0, 1], n)], n, commute=False)]
+    pk += [pyqc.R_x(q, n) for q in range(n)] + [pyqc.CZ([0, n - 1], n)]
+    og = [OC.R_y(q, n) for q in range(n)] + [OC.CNOT((q, q + 1), n) for q in range(n - 1)]
+    og += [OC.shared_parameter([OC.R_x(0, n), OC.R_y(0, n), OC.R_zz((0, 1), n)], n, commute=False)]
+    og += [OC.R_x(q, n) for q in range(n)] + [OC.CZ((0, n - 1), n)]
+    qc = pyqc.PQC(n)
+    qc.add_layer(pk)
+    return qc, og
+
+
+@pytest.mark.parametrize("case", ["xxz", "noncommuting"])
+def test_commute_false_derivatives_match_oracle(pyqc, case):
+    """shared_parameter(commute=False) (gates.py:458-466): XXZ blocks (R_yy,
+    R_xx on disjoint pairs: symmetric, commuting strings) compile to the
+    generator sum; a genuinely non-commuting block gets a plan of its own
+    (the circuit with conj(U) and the interleaved D_k inserted) whose
+    tangent, evolved part by part through the sweep order, is the oracle's
+    derivative state (dense restatement of the reference's formula)."""
+    from pyramaterised_amd import engine as E
+    n = 6 if case == "xxz" else 5
+    qc, og = (_xxz_pair if case == "xxz" else _noncommuting)(pyqc, n)
+    rng = np.random.default_rng(11)
+    th = rng.random(qc.n_true_params) * 2 * np.pi
+    ref = OS.derivative_states(og, th, n)
+    specials = qc.special_slots()
+    if case == "xxz":
+        assert specials == []
+    else:
+        assert len(specials) == 1
+    ops = qc.ops(True)
+    plan = E.Plan(n, ops, tile_bits=4, low_bits=2)
+    parts = _gen_parts(plan, ops)
+    for t in range(qc.n_true_params):
+        if t in specials:
+            sops = qc.special_ops(t)
+            sp = E.Plan(n, sops, tile_bits=4, low_bits=2)
+            sparts = _gen_parts(sp, sops)
+            assert sp.n_tangents == 1 and sp.tangent_parts(0) == len(sparts[0]) == 3
+            d = _derivative_by_parts(sp, sops, n, th, 0, sparts)
+        else:
+            d = _derivative_by_parts(plan, ops, n, th, t, parts)
+        assert np.max(np.abs(d - ref[t])) < 1e-12, t
