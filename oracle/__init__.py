@@ -16,14 +16,17 @@ Pinning (each function's docstring names its own pin):
     (tests.py:192-213), NPQC QFIM == identity (tests.py:103-128), Bell Q == 1
     (tests.py:284-295), fixed_R_y(pi/2)|0> and H.H|0> == |0> (tests.py:64-87);
   * brute force: full 2^N operators built with np.kron at tiny N
-    (``sim.dense_unitary``) against the einsum simulator;
+    (``sim.dense_unitary``) against the einsum simulator; ALLTOALL against a
+    product of CNOT permutation matrices built from bit flips;
   * closed forms and invariants: Bell/product-state purities, unitarity,
     central finite differences of the state for the derivative states, the
     fSim gate's generator form exp(-i theta (XX + YY) / 2) with the phase on
-    |11>.
+    |11>, sqrt(iSWAP)^2 = iSWAP = exp(i pi/4 (XX + YY)), R_PP(t) =
+    exp(-i t/2 P(x)P), the fSim-family derivative matrices (central
+    differences of their differentiated blocks; the kept entries are fSim's
+    own), I's derivative -i/2 psi and new_measure's capped eigenvalue sum.
 
 Parity unpinned (no reference output and no mathematical fixed point pins
-them): ``gates.sqrt_h`` (element-wise square root applied to a QuTiP Qobj at
-gates.py:239-242) and the fSim "derivative" matrices (gates.py:609-648), which
-the oracle restates as written but nothing checks independently.
+it): ``gates.sqrt_h`` (element-wise square root applied to a QuTiP Qobj at
+gates.py:239-242; only QuTiP's np.sqrt-on-Qobj semantics could pin it).
 """

@@ -11,7 +11,10 @@ Conventions (QuTiP 4.7, which the reference calls):
 
 Pins: the qg-circuit energy and the NPQC identity in ``tests/test_oracle.py``
 exercise rx/ry/rz/cnot/cz; unitarity and the textbook identities
-(H = X.Ry(pi/2), H.H = I, S.S = Z, T^2 = S, R_zz diagonal) are checked there too.
+(H = X.Ry(pi/2), H.H = I, S.S = Z, T^2 = S, R_zz diagonal, rotations and
+R_PP as matrix exponentials, sqrt(iSWAP)^2 = iSWAP, fSim's generator form)
+and central differences of the fSim-family derivative matrices are checked
+there too.
 """
 import numpy as np
 
@@ -99,7 +102,9 @@ def fsim(t, f):
 
 
 def fsim_d_theta(t, f):
-    """fsim_gate_d_theta (gates.py:619-627), as written (not dU/dtheta). Unpinned."""
+    """fsim_gate_d_theta (gates.py:619-627), as written (not dU/dtheta): the
+    exchange block differentiated, the corners fSim's.  Pinned by central
+    differences of fsim."""
     return np.array([[1, 0, 0, 0],
                      [0, -np.sin(t), -1j * np.cos(t), 0],
                      [0, -1j * np.cos(t), -np.sin(t), 0],
@@ -107,7 +112,8 @@ def fsim_d_theta(t, f):
 
 
 def fsim_d_phi(t, f):
-    """fsim_gate_d_phi (gates.py:640-648), as written. Unpinned."""
+    """fsim_gate_d_phi (gates.py:640-648), as written: the |11> entry
+    differentiated, the rest fSim's.  Pinned by central differences."""
     return np.array([[1, 0, 0, 0],
                      [0, np.cos(t), -1j * np.sin(t), 0],
                      [0, -1j * np.sin(t), np.cos(t), 0],
@@ -120,7 +126,8 @@ def fixed_fsim(t):
 
 
 def fixed_fsim_d_theta(t):
-    """fixed_fsim_gate_d_theta (gates.py:729-737), as written. Unpinned."""
+    """fixed_fsim_gate_d_theta (gates.py:729-737), as written: the exchange
+    block differentiated, both corners 1.  Pinned by central differences."""
     m = fsim_d_theta(t, 0.0)
     m[3, 3] = 1
     return m

@@ -254,3 +254,102 @@ def test_noncommuting_shared_reduces_for_diagonal_block():
     da = S.apply_deriv(S.apply_prims(psi, a.prims(t), n), a.deriv(t), n)
     db = S.apply_deriv(S.apply_prims(psi, b.prims(t), n), b.deriv(t), n)
     assert np.max(np.abs(da - db)) < 1e-14
+
+
+def test_sqrt_iswap_pins():
+    """sqrtiSWAP (gates.py:344-349, QuTiP sqrtiswap): its square is iSWAP,
+    and it is exp(i pi/8 (XX + YY)), the half-way point of the XY exchange
+    that reaches iSWAP at pi/4."""
+    import scipy.linalg
+    iswap = np.array([[1, 0, 0, 0], [0, 0, 1j, 0], [0, 1j, 0, 0], [0, 0, 0, 1]])
+    assert np.allclose(OG.SQRT_ISWAP @ OG.SQRT_ISWAP, iswap, atol=1e-15)
+    X, Y = OG.PAULI["x"], OG.PAULI["y"]
+    xy = np.kron(X, X) + np.kron(Y, Y)
+    assert np.allclose(OG.SQRT_ISWAP, scipy.linalg.expm(1j * np.pi / 8 * xy), atol=1e-15)
+    assert np.allclose(iswap, scipy.linalg.expm(1j * np.pi / 4 * xy), atol=1e-15)
+
+
+def test_rr_family_is_exp_of_pauli_pair():
+    """R_xx / R_yy / R_zz (gates.py:509-517) = exp(-i t/2 P(x)P)."""
+    import scipy.linalg
+    for p in "xyz":
+        PP = np.kron(OG.PAULI[p], OG.PAULI[p])
+        for t in (0.0, 0.37, 2.1, -1.3, np.pi):
+            assert np.allclose(OG.rpp(t, p), scipy.linalg.expm(-0.5j * t * PP), atol=1e-14), (p, t)
+
+
+def test_fsim_derivative_matrices_against_central_differences():
+    """The fSim-family "derivative" matrices as the reference writes them
+    (gates.py:609-648, 719-737): in fsim_gate_d_theta the exchange block is
+    d/dtheta of fSim's, and the corners are fSim's own (1 and e^{-i phi},
+    kept, not differentiated); in fsim_gate_d_phi the |11> entry is
+    d/dphi of fSim's and the rest is fSim's; fixed_fsim_gate_d_theta is the
+    exchange-block derivative with both corners 1.  Central differences,
+    h = 1e-6."""
+    h = 1e-6
+    for t, f in ((0.3, 0.9), (2.2, -1.1), (np.pi / 2, np.pi / 3), (5.0, 4.0)):
+        fs = OG.fsim(t, f)
+        dt = (OG.fsim(t + h, f) - OG.fsim(t - h, f)) / (2 * h)
+        df = (OG.fsim(t, f + h) - OG.fsim(t, f - h)) / (2 * h)
+        m = OG.fsim_d_theta(t, f)
+        assert np.allclose(m[1:3, 1:3], dt[1:3, 1:3], atol=1e-9)
+        assert m[0, 0] == 1 and m[3, 3] == fs[3, 3]
+        off = np.ones((4, 4), bool)
+        off[0, 0] = off[3, 3] = False
+        off[1:3, 1:3] = False
+        assert np.all(m[off] == 0)
+        m2 = OG.fsim_d_phi(t, f)
+        assert abs(m2[3, 3] - df[3, 3]) < 1e-9
+        keep = np.ones((4, 4), bool)
+        keep[3, 3] = False
+        assert np.array_equal(m2[keep], fs[keep])
+        m3 = OG.fixed_fsim_d_theta(t)
+        dfix = (OG.fixed_fsim(t + h) - OG.fixed_fsim(t - h)) / (2 * h)
+        assert np.allclose(m3[1:3, 1:3], dfix[1:3, 1:3], atol=1e-9)
+        assert m3[0, 0] == 1 and m3[3, 3] == 1 and np.all(m3[off] == 0)
+
+
+def test_alltoall_is_the_product_over_ordered_pairs():
+    """ALLTOALL (gates.py:382-401) applies the entangler on every ordered pair
+    of permutations(range(N), 2), the first listed first: against a product
+    of CNOT permutation matrices built from bit flips (no embed)."""
+    from itertools import permutations
+    n = 4
+    dim = 2 ** n
+    want = np.eye(dim, dtype=complex)
+    for a, b in permutations(range(n), 2):
+        P = np.zeros((dim, dim))
+        for i in range(dim):
+            j = i ^ (1 << (n - 1 - b)) if (i >> (n - 1 - a)) & 1 else i
+            P[j, i] = 1
+        want = P @ want
+    got = S.dense_unitary([C.ALLTOALL(C.CNOT, n)], [], n)
+    assert np.array_equal(got, want)
+    # the order matters: the reversed product differs
+    rev = np.eye(dim, dtype=complex)
+    for a, b in reversed(list(permutations(range(n), 2))):
+        P = np.zeros((dim, dim))
+        for i in range(dim):
+            j = i ^ (1 << (n - 1 - b)) if (i >> (n - 1 - a)) & 1 else i
+            P[j, i] = 1
+        rev = P @ rev
+    assert not np.array_equal(got, rev)
+
+
+def test_identity_gate_and_new_measure_closed_forms():
+    """I (gates.py:150-156): the state is unchanged and the derivative state
+    is -i/2 psi (PRot.derivative with the identity, gates.py:133-138).
+    new_measure (measure.py:89-99): the sum of eigenvalues capped at 1."""
+    n = 3
+    gates = [C.R_y(0, n), C.CNOT((0, 2), n), C.I(1, n), C.R_x(2, n)]
+    th = np.array([0.7, 1.9, 2.4])
+    psi = S.run(gates, th, n)
+    assert np.allclose(psi, S.run([gates[0], gates[1], gates[3]], th[[0, 2]], n), atol=1e-15)
+    d = S.derivative_states(gates, th, n)
+    assert np.allclose(d[1], -0.5j * psi, atol=1e-15)
+    rng = np.random.default_rng(4)
+    Q, _ = np.linalg.qr(rng.normal(size=(4, 4)))
+    ev = np.array([0.25, 3.0, 1.0, 0.0])
+    F = Q @ np.diag(ev) @ Q.T
+    assert abs(M.new_measure(F) - 2.25) < 1e-12
+    assert abs(M.new_measure(np.eye(5)) - 5) < 1e-12
