@@ -57,7 +57,8 @@ enum pqs_op_kind {
   PQS_OP_RXX = 7,    /* cos(a/2) I - i sin(a/2) X(x)X                        */
   PQS_OP_RYY = 8,    /* ... Y(x)Y                                            */
   PQS_OP_RZZ = 9,    /* ... Z(x)Z                                            */
-  PQS_OP_FSIM = 10,  /* fsim(theta = angle(slot), phi = params[slot2])       */
+  PQS_OP_FSIM = 10,  /* fsim(theta = angle(slot), phi = params[slot2]; 0 if
+                        slot2 = -1, i.e. fixed_fSim)                          */
   /* Derivative ops: applied only by the instance whose tangent index equals
    * ``tangent`` (a no-op for every other instance).  Consecutive PQS_OP_GEN
    * terms with the same tangent form one generator D = sum_t coef_t P_t,

@@ -67,7 +67,6 @@ static int validate(int n, const pqs_op &o, int idx) {
     if (o.q1 == o.q0) return bad("q0 == q1");
   }
   if (has_angle(o.kind) && o.slot < 0 && o.scale != 0.0) return bad("angle op without a slot");
-  if (o.kind == PQS_OP_FSIM && o.slot2 < 0) return bad("fSim without a phi slot");
   if (o.kind == PQS_OP_GEN || o.kind == PQS_OP_GEN_FSIM) {
     if (o.tangent < 0) return bad("derivative op without a tangent index");
     if (o.kind == PQS_OP_GEN && (o.pauli < 0 || o.pauli > 3)) return bad("pauli must be 0..3");
