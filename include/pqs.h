@@ -227,6 +227,25 @@ int pqs_batch_table(pqs_ctx *ctx, pqs_plan *plan, void *states_dev, int64_t stat
 int pqs_run_table(pqs_ctx *ctx, pqs_plan *plan, const void *table_dev, int64_t batch,
                   const void *init_table_dev, void *stream);
 
+/* pqs_run_batch (plain evolution, no tangents) without any allocation.  No
+ * counterpart in the reference: it serves callers that cannot allocate
+ * device memory -- code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc.  The instance table is written by a
+ * kernel into a file-scope device array of the library (one per GPU, held
+ * until the process ends), so the call allocates nothing, copies nothing
+ * from the host and only launches kernels on `stream`.  Limits: batch <=
+ * 16384 instances (PQS_ERR_ARG beyond); the plan must have run once on
+ * ctx's device (pqs_run_batch / pqs_run / pqs_batch_table upload its device
+ * image; PQS_ERR_ARG otherwise) and must carry no tangents
+ * (PQS_ERR_UNSUPPORTED); a rolling-layout plan needs alt_dev (batch *
+ * state_stride amplitudes, caller-owned) and, with init 0, a src_dev that is
+ * not the output.  Every call on one GPU shares the one array: calls that
+ * overlap on one GPU (several streams or host threads) race and must be
+ * serialised by the caller.  Arguments otherwise as pqs_run_batch. */
+int pqs_run_batch_noalloc(pqs_ctx *ctx, pqs_plan *plan, void *states_dev, int64_t state_stride, int64_t batch,
+                          const double *params_dev, int64_t param_stride, int32_t init, const void *src_dev,
+                          void *alt_dev, const void *init_table_dev, void *stream);
+
 /* All derivative states in one pass: psi_dev gets U|psi0>, jac_dev (row t =
  * tangent t, n_tangents rows of 2^N) gets every d psi / d theta_t.  Tangent t
  * is spawned from the plain state at the sweep that holds its generator and

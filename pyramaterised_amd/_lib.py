@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libpqs.so")
 
 PQS_OK = 0
+PQS_ERR_ARG, PQS_ERR_HIP, PQS_ERR_UNSUPPORTED, PQS_ERR_NOMEM, PQS_ERR_NODEVICE = -1, -2, -3, -4, -5
 PQS_C128, PQS_C64 = 0, 1
 # pqs_plan_opts.flags
 PQS_PLAN_LANES_ASCENDING, PQS_PLAN_NO_SOFT_REGS, PQS_PLAN_JIT_PERSISTENT, PQS_PLAN_ROLLING = 1, 2, 4, 8
@@ -60,6 +61,7 @@ SIGNATURES = {
     "pqs_plan_tangent_parts": (C.c_int, [vp, i32, C.POINTER(i32)]),
     "pqs_run": (C.c_int, [vp, vp, C.POINTER(Instance), i64, vp, vp]),
     "pqs_run_batch": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, vp, i32, vp, vp, vp]),
+    "pqs_run_batch_noalloc": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, i32, vp, vp, vp, vp]),
     "pqs_jacobian": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, vp]),
     "pqs_inner": (C.c_int, [vp, i32, vp, i64, vp, vp, i64, vp, vp]),
     "pqs_mw": (C.c_int, [vp, i32, vp, i32, i64, vp, vp, vp]),

@@ -1374,6 +1374,32 @@ __global__ __launch_bounds__(256) void k_add_rows(const AccRow *__restrict__ acc
   }
 }
 
+// Instance tables of pqs_run_batch_noalloc live in this file-scope device
+// array: one per GPU, shared by every call on that GPU (calls that overlap
+// on one GPU race for it).  Host code never passes its name to a kernel: it
+// uses the per-device address from hipGetSymbolAddress (pqs_ctx_create).
+constexpr int64_t kNoallocMaxBatch = 16384;
+static __device__ __align__(256) unsigned char scratch[kNoallocMaxBatch * sizeof(InstDev)];
+
+// Instance table of a strided batch, written on the device (no host copy).
+__global__ __launch_bounds__(256) void k_fill_batch_table(InstDev *__restrict__ tab, char *states, int64_t state_bytes,
+                                                          int64_t batch, const double *params, int64_t param_stride,
+                                                          int32_t init, const void *src, char *alt,
+                                                          int32_t pingpong) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= batch) return;
+  InstDev d;
+  d.state = states + i * state_bytes;
+  d.src = src;
+  d.params = params ? params + i * param_stride : nullptr;
+  d.alt = pingpong ? alt + i * state_bytes : nullptr;
+  d.tangent = -1;
+  d.init = init;
+  d.start_sweep = 0;
+  d.pingpong = pingpong;
+  tab[i] = d;
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -1442,6 +1468,7 @@ struct pqs_ctx {
   size_t ws_bytes = 0;
   hipStream_t ws_stream = nullptr;  // stream of the last call that used ws
   hipEvent_t ws_event = nullptr;
+  void *scratch = nullptr;  // this device's address of the file-scope scratch array
   void *pin = nullptr;
   size_t pin_bytes = 0;
   hipEvent_t pin_event = nullptr;
@@ -1708,6 +1735,23 @@ int pqs_ctx_create(int device, pqs_ctx **out) {
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
     c->max_lds = v;
   std::snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
+  {
+    // the scratch array's address on this device, looked up once per device
+    // with the device current (DeviceGuard above)
+    static std::mutex mu;
+    static void *addr[64] = {nullptr};
+    std::lock_guard<std::mutex> lk(mu);
+    if (device < 64 && !addr[device]) {
+      void *ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr) {
+        delete c;
+        return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+      }
+      addr[device] = ptr;
+    }
+    c->scratch = device < 64 ? addr[device] : nullptr;
+  }
   if (hipEventCreateWithFlags(&c->pin_event, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ws_event, hipEventDisableTiming) != hipSuccess) {
     if (c->pin_event) (void)hipEventDestroy(c->pin_event);
@@ -1972,6 +2016,39 @@ int pqs_run_table(pqs_ctx *ctx, pqs_plan *p, const void *table, int64_t batch, c
   hipStream_t st = (hipStream_t)stream;
   for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
     int rc = launch_sweep(ctx, p, s, (const InstDev *)table, batch, init_tab, st);
+    if (rc) return rc;
+  }
+  return PQS_OK;
+}
+
+int pqs_run_batch_noalloc(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride, int64_t batch,
+                          const double *params, int64_t param_stride, int32_t init, const void *src, void *alt,
+                          const void *init_tab, void *stream) {
+  if (!ctx || !p || !states) return fail(PQS_ERR_ARG, "NULL ctx, plan or states");
+  if (batch < 0) return fail(PQS_ERR_ARG, "negative batch");
+  if (batch > kNoallocMaxBatch)
+    return fail(PQS_ERR_ARG, "batch exceeds the no-allocation limit of 16384 instances: split it");
+  if (batch == 0) return PQS_OK;
+  if (state_stride < (int64_t(1) << p->n)) return fail(PQS_ERR_ARG, "state stride below 2^N");
+  if (init < 0 || init > 2) return fail(PQS_ERR_ARG, "init must be 0, 1 or 2");
+  if (init == 2 && !init_tab) return fail(PQS_ERR_ARG, "product init needs an init table");
+  if (p->n_tangents > 0) return fail(PQS_ERR_UNSUPPORTED, "no-allocation runs take plans without tangents");
+  if (!ctx->scratch) return fail(PQS_ERR_UNSUPPORTED, "no scratch array on this device");
+  if (!p->d_ops || p->dev_device != ctx->device)
+    return fail(PQS_ERR_ARG, "plan not on this device: run it once with pqs_run_batch first");
+  if (p->rolling) {
+    if (!alt) return fail(PQS_ERR_ARG, "rolling-layout plan needs the alt buffer (batch * state_stride amplitudes)");
+    if (init == 0 && (!src || src == states)) return fail(PQS_ERR_ARG, "rolling-layout plan: src must not be the output");
+  }
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  InstDev *tab = static_cast<InstDev *>(ctx->scratch);
+  const int64_t sbytes = state_stride * int64_t(amp_bytes(p->dtype));
+  hipLaunchKernelGGL(k_fill_batch_table, dim3(grid_for(batch, 256)), dim3(256), 0, st, tab, (char *)states, sbytes,
+                     batch, params, param_stride, init, src, (char *)alt, p->rolling ? 1 : 0);
+  HIPCHECK(hipGetLastError());
+  for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
+    const int rc = launch_sweep(ctx, p, s, tab, batch, init_tab, st);
     if (rc) return rc;
   }
   return PQS_OK;
