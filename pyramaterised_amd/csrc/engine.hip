@@ -1169,14 +1169,35 @@ constexpr int SYT = 128;
 typedef double d4_t __attribute__((ext_vector_type(4)));
 constexpr int MFK = 16, MFLD = MFK + 2;
 
-__global__ __launch_bounds__(256, 2) void k_syrk_mfma(const double *__restrict__ R, int64_t ld, int64_t P, int64_t K,
-                                                   int64_t kchunk, const int2 *__restrict__ tiles,
-                                                   double *__restrict__ part) {
+// Work of one block: the XCD-aware map from the linear block index.  The
+// dispatcher deals consecutive workgroups round-robin to the 8 XCDs, each
+// with its own L2; chunk c of the K split belongs to XCD c % 8, and every
+// tile of a chunk runs on that XCD, so the chunk's rows are fetched from HBM
+// once and re-read (by the other tiles sharing them) from that XCD's L2.
+// Blocks past an XCD's share (ks not a multiple of 8) return at once.
+struct SyrkWork {
+  int tile, chunk;
+};
+__device__ __forceinline__ SyrkWork syrk_work(int64_t L, int64_t T, int64_t ks) {
+  const int64_t x = L & 7, j = L >> 3;
+  const int64_t n_x = (ks - x + 7) >> 3;  // chunks x, x + 8, ... below ks
+  if (j >= n_x * T) return SyrkWork{-1, -1};
+  return SyrkWork{int(j % T), int(x + 8 * (j / T))};
+}
+
+// TIn = double (complex128 Jacobian) or float (complex64: operands widened
+// to double in registers, so products and sums are the FP64 path's).
+template <typename TIn>
+__global__ __launch_bounds__(256, 2) void k_syrk_mfma(const TIn *__restrict__ R, int64_t ld, int64_t P, int64_t K,
+                                                      int64_t kchunk, int64_t ks, const int2 *__restrict__ tiles,
+                                                      int64_t T, double *__restrict__ part) {
   __shared__ double sa[SYT * MFLD];
   __shared__ double sb[SYT * MFLD];
-  const int2 tile = tiles[blockIdx.x];
+  const SyrkWork wk = syrk_work(blockIdx.x, T, ks);
+  if (wk.tile < 0) return;
+  const int2 tile = tiles[wk.tile];
   const int64_t i0 = int64_t(tile.x) * SYT, j0 = int64_t(tile.y) * SYT;
-  const int64_t k0 = int64_t(blockIdx.y) * kchunk;
+  const int64_t k0 = int64_t(wk.chunk) * kchunk;
   const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
   const int t = threadIdx.x, l = t & 63, w = t >> 6;
   const int qr = (w >> 1) * 64, qc = (w & 1) * 64;
@@ -1185,17 +1206,25 @@ __global__ __launch_bounds__(256, 2) void k_syrk_mfma(const double *__restrict__
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
-  // staging: element e = t + 256 u is row e / 8, doubles 2 (e % 8) + {0, 1}
+  // staging: element e = t + 256 u is row e / 8, values 2 (e % 8) + {0, 1}
   // of the step's 16 (K, kchunk and the steps are even, so a pair is
   // either inside [k0, k1) or outside it)
   double2 pa[4], pb[4];
+  auto ld2 = [&](int64_t row, int64_t k) -> double2 {
+    if constexpr (sizeof(TIn) == 8) {
+      return *reinterpret_cast<const double2 *>(R + row * ld + k);
+    } else {
+      const float2 f = *reinterpret_cast<const float2 *>(R + row * ld + k);
+      return double2{double(f.x), double(f.y)};
+    }
+  };
   auto gload = [&](int64_t kk) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u, row = e >> 3, c2 = e & 7;
       const int64_t k = kk + 2 * c2, ra = i0 + row, rb = j0 + row;
-      pa[u] = (ra < P && k < k1) ? *reinterpret_cast<const double2 *>(R + ra * ld + k) : double2{0.0, 0.0};
-      pb[u] = (rb < P && k < k1) ? *reinterpret_cast<const double2 *>(R + rb * ld + k) : double2{0.0, 0.0};
+      pa[u] = (ra < P && k < k1) ? ld2(ra, k) : double2{0.0, 0.0};
+      pb[u] = (rb < P && k < k1) ? ld2(rb, k) : double2{0.0, 0.0};
     }
   };
   if (k0 < k1) gload(k0);
@@ -1222,7 +1251,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk_mfma(const double *__restrict__
         for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
   }
-  double *out = part + int64_t(blockIdx.y) * P * P;
+  double *out = part + int64_t(wk.chunk) * P * P;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -1255,23 +1284,6 @@ __global__ void k_gram_reduce(const double2 *__restrict__ part, int64_t mn, int6
     a.y += part[z * mn + e].y;
   }
   out[e] = a;
-}
-
-__global__ void k_qfim_combine(const double2 *__restrict__ G, const double2 *__restrict__ bvec, int64_t P,
-                               double *__restrict__ F) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= P * P) return;
-  int64_t p = e / P, q = e - p * P;
-  if (p > q) {  // mirror the upper triangle (measure.py:66-70)
-    const int64_t t = p;
-    p = q;
-    q = t;
-  }
-  const double2 g = G[p * P + q];
-  const double2 bp = bvec[p], bq = bvec[q];
-  // Re(conj(bp) * bq)
-  const double rhs = bp.x * bq.x + bp.y * bq.y;
-  F[e] = 4 * (g.x - rhs);
 }
 
 // F from the upper triangle of the real Gram C = Re(J J^dag) (column-major,
@@ -2318,22 +2330,27 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
   if (P <= 0 || dim <= 0) return fail(PQS_ERR_ARG, "bad sizes");
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == PQS_C128) {
+  {
     // Re(J J^dag) = R R^T, R = J as a real P x 2dim matrix: the upper
-    // tiles of the symmetric product on the matrix cores, split over K
+    // tiles of the symmetric product on the FP64 matrix cores, split over K
     // (rocBLAS dgemm on these tall-skinny shapes ran 2-10x slower, erratic
-    // in P, and computes both triangles)
+    // in P, and computes both triangles).  complex64 Jacobians are widened
+    // to double in registers: the same products and sums.
     const int64_t K = 2 * dim;
     const int64_t nt = (P + SYT - 1) / SYT;
     std::vector<int2> tl;
     for (int64_t a = 0; a < nt; ++a)
       for (int64_t b = a; b < nt; ++b) tl.push_back(int2{int(a), int(b)});
     const int64_t T = int64_t(tl.size());
+    // enough blocks for two per CU, the split a multiple of the 8 XCDs
     int64_t ks = (2 * int64_t(ctx->n_cu) + T - 1) / T;
+    ks = (ks + 7) / 8 * 8;
     ks = std::max<int64_t>(1, std::min<int64_t>(ks, K / 256));
     int64_t kchunk = (K + ks - 1) / ks;
     kchunk = (kchunk + MFK - 1) / MFK * MFK;
     ks = (K + kchunk - 1) / kchunk;
+    const int64_t grid = 8 * ((ks + 7) / 8) * T;
+    if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "QFIM too large");
     const size_t part_b = std::max(gram_part_bytes(ctx, 1, P, dim), size_t(ks * P * P) * sizeof(double));
     const size_t off_c = (part_b + 255) & ~size_t(255);
     const size_t off_b = (off_c + size_t(P * P) * sizeof(double) + 255) & ~size_t(255);
@@ -2350,8 +2367,12 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
     if (rc) return rc;
     // the partials reuse the workspace head once the b vector is reduced
     // (same stream, so in order)
-    hipLaunchKernelGGL(k_syrk_mfma, dim3(unsigned(T), unsigned(ks)), dim3(256), 0, st, (const double *)jac, K, P, K,
-                       kchunk, d_tiles, (double *)ws);
+    if (dtype == PQS_C128)
+      hipLaunchKernelGGL(k_syrk_mfma<double>, dim3(unsigned(grid)), dim3(256), 0, st, (const double *)jac, K, P, K,
+                         kchunk, ks, d_tiles, T, (double *)ws);
+    else
+      hipLaunchKernelGGL(k_syrk_mfma<float>, dim3(unsigned(grid)), dim3(256), 0, st, (const float *)jac, K, P, K,
+                         kchunk, ks, d_tiles, T, (double *)ws);
     HIPCHECK(hipGetLastError());
     hipLaunchKernelGGL(k_syrk_reduce, dim3(grid_for(P * P, 256)), dim3(256), 0, st, (const double *)ws, P, ks, C);
     HIPCHECK(hipGetLastError());
@@ -2359,23 +2380,6 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
     HIPCHECK(hipGetLastError());
     return PQS_OK;
   }
-  // workspace: [partials | G (P x P) | b (P)]
-  const size_t part_b = std::max(gram_part_bytes(ctx, P, P, dim), gram_part_bytes(ctx, 1, P, dim));
-  const size_t off_g = (part_b + 255) & ~size_t(255);
-  const size_t off_b = (off_g + size_t(P * P) * sizeof(double2) + 255) & ~size_t(255);
-  int rc = ws_reserve(ctx, off_b + size_t(P) * sizeof(double2), st);
-  if (rc) return rc;
-  char *ws = (char *)ctx->ws;
-  double2 *part = (double2 *)ws;
-  double2 *G = (double2 *)(ws + off_g);
-  double2 *bv = (double2 *)(ws + off_b);
-  rc = gram_impl(ctx, dtype, jac, dim, P, jac, dim, P, dim, G, part, st);
-  if (rc) return rc;
-  rc = gram_impl(ctx, dtype, psi, dim, 1, jac, dim, P, dim, bv, part, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_qfim_combine, dim3(grid_for(P * P, 256)), dim3(256), 0, st, G, bv, P, F);
-  HIPCHECK(hipGetLastError());
-  return PQS_OK;
 }
 
 int pqs_expect_pauli(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t batch, const uint64_t *xm,
