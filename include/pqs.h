@@ -304,6 +304,14 @@ int pqs_expect_pauli(pqs_ctx *ctx, int32_t dtype, const void *states_dev, int32_
 int pqs_histogram(pqs_ctx *ctx, const double *values_dev, int64_t n, const double *edges_dev,
                   int32_t n_bins, int64_t *counts_dev, void *stream);
 
+/* out_b = U x_b for a dense operator U (dim x dim complex128, row-major, device)
+ * on a batch of states (row stride `stride` amplitudes, dtype of the
+ * states; out must not alias them), dim <= 2^14.  Applies gates that have no
+ * local form: ARBGATE's U = expm(-i theta H) and its derivative matrix
+ * (reference gates.py:407-435), computed on the host. */
+int pqs_apply_dense(pqs_ctx *ctx, int32_t dtype, const double *u_dev, int64_t dim, const void *states_dev,
+                    int64_t stride, int64_t batch, void *out_dev, void *stream);
+
 /* |z|^2 of complex128 values: out[i] = re^2 + im^2 (fidelity from overlap). */
 int pqs_abs2(pqs_ctx *ctx, const double *z_dev, int64_t n, double *out_dev, void *stream);
 

@@ -351,3 +351,133 @@ def xxz(p, n, commute=False):
         gates.append(shared_parameter([R_zz(pr, n) for pr in even], n))
         gates.append(shared_parameter([R_yy(pr, n) for pr in even] + [R_xx(pr, n) for pr in even], n, commute))
     return gates
+
+
+# --------------------------------------------------- generate_circuit types
+# Restated from templates.py:97-353 (gate order is the parameter-binding
+# order, so it is the contract); tests build the package's circuits with
+# generate_circuit (templates.py:361-429) and compare states.
+
+def _ent(name):
+    return {"cnot": CNOT, "cphase": CZ, "cz": CZ, "sqrtiswap": sqrtiSWAP}[name]
+
+
+def circuit_1(p, n):
+    """circuit_1_layers (templates.py:112-117): R_x all, R_y all, per layer."""
+    return [g for _ in range(p) for g in [R_x(i, n) for i in range(n)] + [R_y(i, n) for i in range(n)]]
+
+
+def circuit_2(p, n, ent="cnot"):
+    """circuit_2_layers (templates.py:120-126): R_x all, R_z all, CHAIN."""
+    return [g for _ in range(p)
+            for g in [R_x(i, n) for i in range(n)] + [R_z(i, n) for i in range(n)] + [CHAIN(_ent(ent), n)]]
+
+
+def circuit_9(p, n, ent="cphase"):
+    """circuit_9_layers (templates.py:129-135): H all, CHAIN, R_x all."""
+    return [g for _ in range(p)
+            for g in [H(i, n) for i in range(n)] + [CHAIN(_ent(ent), n)] + [R_x(i, n) for i in range(n)]]
+
+
+def qg_circuit(p, n, ent="cnot"):
+    """qg_circuit_layers (templates.py:138-150): fixed R_y(pi/4) all, then p x
+    [R_z all, CHAIN, R_x all, CHAIN, R_z all, CHAIN]."""
+    gates = [fixed_R_y(i, n, np.pi / 4) for i in range(n)]
+    for _ in range(p):
+        for rot in (R_z, R_x, R_z):
+            gates += [rot(i, n) for i in range(n)] + [CHAIN(_ent(ent), n)]
+    return gates
+
+
+def y_cphase(p, n):
+    """y_CPHASE_layers (templates.py:173-180)."""
+    return [g for _ in range(p) for g in [R_y(i, n) for i in range(n)] + [CHAIN(CZ, n)]]
+
+
+def double_y_cphase(p, n):
+    """double_y_CPHASE_layers (templates.py:182-189): R_y all twice, CHAIN(CPHASE)."""
+    return [g for _ in range(p)
+            for g in [R_y(i, n) for i in range(n)] + [R_y(i, n) for i in range(n)] + [CHAIN(CZ, n)]]
+
+
+def modified_tfim(p, n):
+    """modified_TFIM_layers (templates.py:207-217): RR_block(R_zz), shared R_x,
+    shared R_z, per layer."""
+    gates = []
+    for _ in range(p):
+        gates += [RR_block("z", n), shared_parameter([R_x(i, n) for i in range(n)], n),
+                  shared_parameter([R_z(i, n) for i in range(n)], n)]
+    return gates
+
+
+def theta_block(a, b, n):
+    """gen_theta_block (templates.py:260-274): sqrtiSWAP, shared
+    [negative_R_z(a), offset_R_z(b, pi)], sqrtiSWAP, fixed_R_z(b, pi)."""
+    return [sqrtiSWAP((a, b), n), shared_parameter([negative_R_z(a, n), offset_R_z(b, n, np.pi)], n),
+            sqrtiSWAP((a, b), n), fixed_R_z(b, n, np.pi)]
+
+
+def fermionic(p, n):
+    """fermionic_circuit_layers (templates.py:282-304): per repetition a
+    diamond of theta-block layers; layer m (1-based, m = 1..n//2, then back
+    down to 1) pairs the qubits n//2-m .. n//2+m-1 (0-based) in neighbours."""
+    gates = []
+    widths = list(range(1, n // 2 + 1))
+    widths = widths + widths[::-1][1:]
+    for _ in range(p):
+        for m in widths:
+            qs = list(range(n // 2 - m, n // 2 + m))  # 0-based qubits of this layer
+            for i in range(0, len(qs) - 1, 2):
+                gates += theta_block(qs[i], qs[i + 1], n)
+    return gates
+
+
+def fsim_circuit(p, n, rotator="y", fixed_gate=False):
+    """fSim_circuit_layers (templates.py:307-353): rotations on every qubit,
+    then fSim on (i, i+1 mod n) for i of the layer's parity (even n), or, for
+    odd n, one qubit left out per layer (index l mod n), pairs of the rest in
+    order, the first and last paired when the left-out index is odd, and a
+    closing rotation on the left-out qubit."""
+    rot = {"x": R_x, "y": R_y, "z": R_z}[rotator]
+    ent = fixed_fSim if fixed_gate else fSim
+    gates = []
+    for l in range(p):
+        gates += [rot(i, n) for i in range(n)]
+        if n % 2 == 0:
+            gates += [ent((i, (i + 1) % n), n) for i in range(l % 2, n, 2)]
+        else:
+            skip = l % n
+            rest = [i for i in range(n) if i != skip]
+            pairs = [(rest[0], rest[-1])] if skip % 2 == 1 else []
+            inner = rest[1:-1] if skip % 2 == 1 else rest
+            pairs += [(inner[i], inner[i + 1]) for i in range(0, len(inner) - 1, 2)]
+            gates += [ent(pr, n) for pr in pairs]
+            gates.append(rot(skip, n))
+    return gates
+
+
+def half_filled(n):
+    """The fermionic / XXZ / fSim start state of generate_circuit without
+    shuffling (templates.py:377-380): |1> on the first n//2 qubits, |0> on
+    the rest."""
+    psi = np.array([1.0 + 0j])
+    for q in range(n):
+        psi = np.kron(psi, np.array([0, 1]) if q < n // 2 else np.array([1, 0]))
+    return psi
+
+
+def ARBGATE(H, n):
+    """ARBGATE (gates.py:407-435): U = expm(-i theta H) over all n qubits
+    (scipy.linalg.expm), and the reference's derivative D = -i H / 2 (half
+    of dU/dtheta, as written), applied after U."""
+    import scipy.linalg
+    H = np.asarray(H, dtype=np.complex128)
+    qs = tuple(range(n))
+
+    def prims(t, _p):
+        return [(qs, scipy.linalg.expm(-1j * t * H))]
+
+    def deriv(_t, _p, _w):
+        return [(-0.5j, [(qs, H)])]
+
+    return OGate("ARBGATE", 1, prims, deriv)

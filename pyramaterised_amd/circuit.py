@@ -48,13 +48,23 @@ class PQC:
 
     # ------------------------------------------------------------ set-up
     def set_H(self, H):
-        """circuit.py:25-33: 'ZZ' -> Z_0 Z_1, else a user operator."""
+        """circuit.py:25-33: 'ZZ' -> Z_0 Z_1, else a user operator.  A dense
+        operator is decomposed into Pauli strings once, on first use
+        (``hamiltonian``), not at every ``cost``."""
         if isinstance(H, str) and H == "ZZ":
             s = [0] * self.n_qubits
             s[0] = s[1] = 3
             self.H = PauliSum(self.n_qubits, {tuple(s): 1.0})
         else:
             self.H = H
+        self._H_ps = (self.H, self.H) if isinstance(self.H, PauliSum) else None
+
+    def hamiltonian(self):
+        """self.H as a PauliSum (decomposed once per operator object)."""
+        c = getattr(self, "_H_ps", None)
+        if c is None or c[0] is not self.H:
+            c = self._H_ps = (self.H, self.H if isinstance(self.H, PauliSum) else PauliSum.from_dense(self.H))
+        return c[1]
 
     def set_initial_state(self, state):
         """circuit.py:35-36: tensor([state] * n)."""
@@ -305,10 +315,112 @@ class PQC:
                 out.append(ph)
         return out
 
+    # ------------------------------------------------------------ dense gates
+    # A gate with no local form (ARBGATE: a dense 2^N operator) splits the
+    # circuit into segments of local gates, each run as its own plan, with
+    # the dense operator applied on the GPU in between (pqs_apply_dense).
+    def has_dense(self):
+        return any(getattr(g, "dense", False) for g in self.gates)
+
+    def _segments(self):
+        """[(local gates, dense gate or None)] in gate order."""
+        segs, cur = [], []
+        for g in self.gates:
+            if getattr(g, "dense", False):
+                segs.append((cur, g))
+                cur = []
+            else:
+                cur.append(g)
+        segs.append((cur, None))
+        return segs
+
+    def _segment_plan(self, i, gates, with_generators, dt=None):
+        dt = self.dtype if dt is None else dt
+        key = ("segment", i, with_generators, dt)
+        p = self._plans.get(key)
+        if p is None:
+            ops = []
+            for g in gates:
+                slot = self._gate_slot.get(id(g), -1)
+                ops += g.prim_ops(max(slot, 0))
+                if with_generators and g.param_count > 0:
+                    if getattr(g, "special_derivative", False):
+                        raise NotImplementedError("commute=False blocks that are not generator sums, in a circuit "
+                                                  "with a dense gate")
+                    for w in range(g.param_count):
+                        ops += g.gen_ops(slot + w, slot, which=w + 1)
+            p = E.Plan(self.n_qubits, ops, dtype=dt, **self.plan_options(with_generators, dt))
+            self._plans[key] = p
+        return p
+
+    def _run_dense(self, params):
+        """States (B, 2^N) of a circuit with dense gates, one row per
+        parameter row."""
+        params = np.atleast_2d(np.asarray(params, dtype=np.float64))
+        B = params.shape[0]
+        states = None
+        for i, (gates, dense) in enumerate(self._segments()):
+            if gates:
+                plan = self._segment_plan(i, gates, False)
+                if states is None:
+                    states = E.run(plan, params, self.init())
+                else:
+                    E.run_rows(plan, params, states)
+            if dense is not None:
+                if states is None:
+                    init = self.init()
+                    states = E.run(E.Plan(self.n_qubits, [], dtype=self.dtype), np.zeros((B, 1)), init)
+                t = self._gate_slot[id(dense)]
+                for b in range(B):
+                    states[b] = E.apply_dense(dense.matrix(params[b, t]), states[b])
+        return states
+
+    def _jacobian_dense(self):
+        """(psi, J) of a circuit with dense gates, forward through the
+        segments: rows spawned in a segment (pqs_jacobian from the state at
+        its start) are carried through every later dense gate (U) and
+        segment (plain evolution of the rows); a dense gate's own row is
+        D U psi (gates.py:427-429)."""
+        vec = self._params_vec()
+        P = self.n_true_params
+        dim = 2 ** self.n_qubits
+        dev = E.device()
+        cdt = E.CDTYPE[self.dtype]
+        J = torch.zeros((P, dim), dtype=cdt, device=dev)
+        live = []
+        psi = None
+        for i, (gates, dense) in enumerate(self._segments()):
+            if gates:
+                if live:
+                    rows = J[live].contiguous()
+                    E.run_rows(self._segment_plan(i, gates, False), vec, rows)
+                    J[live] = rows
+                init = self.init() if psi is None else E.Init.from_vector(psi)
+                psi, Jseg = E.jacobian(self._segment_plan(i, gates, True), vec, init)
+                for g in gates:
+                    if g.param_count > 0:
+                        t0 = self._gate_slot[id(g)]
+                        for t in range(t0, t0 + g.param_count):
+                            J[t] = Jseg[t]
+                            live.append(t)
+            if dense is not None:
+                if psi is None:
+                    psi = E.run(E.Plan(self.n_qubits, [], dtype=self.dtype), np.zeros(1), self.init())[0]
+                t = self._gate_slot[id(dense)]
+                th = vec[t]
+                if live:
+                    J[live] = E.apply_dense(dense.matrix(th), J[live].contiguous())
+                J[t] = E.apply_dense(dense.derivative_matrix(th), psi)
+                live.append(t)
+                psi = E.apply_dense(dense.matrix(th), psi)
+        return psi, J
+
     # ------------------------------------------------------------ evolution
     def run(self, angles):
         """circuit.py:118-125: |psi> = U_G ... U_1 |psi0> (one GPU call)."""
         self.set_params(angles=angles)
+        if self.has_dense():
+            return Ket(self._run_dense(self._params_vec())[0], self.n_qubits)
         states = E.run(self.plan(), self._params_vec(), self.init())
         return Ket(states[0], self.n_qubits)
 
@@ -321,6 +433,8 @@ class PQC:
             return torch.empty((0, 2 ** self.n_qubits), dtype=E.CDTYPE[self.dtype], device=E.device())
         last = params[-1].tolist() if not torch.is_tensor(params) else params[-1].cpu().tolist()
         self.set_params(last)
+        if self.has_dense():
+            return self._run_dense(params.cpu().numpy() if torch.is_tensor(params) else params)
         return E.run(self.plan(batch=len(params)), params, self.init())
 
     def update_state(self, angles):
@@ -331,7 +445,7 @@ class PQC:
     def cost(self, angles):
         """circuit.py:132-137: <psi|H|psi>."""
         self.state = self.run(angles=angles)
-        return expect(self.H, self.state)
+        return expect(self.hamiltonian(), self.state)
 
     def fidelity(self, target_state):
         """circuit.py:139-142: |<psi|phi>|^2."""
@@ -357,6 +471,8 @@ class PQC:
         """circuit.py:149-172: the circuit with gate g_on replaced by D*gate,
         run on the initial state (one GPU run with the generator inserted)."""
         t = self._tangent_of(g_on, param)
+        if self.has_dense():
+            return Ket(self._jacobian_dense()[1][t].clone(), self.n_qubits)
         plan, tg = self._tangent_plan(t)
         out = E.run(plan, self._params_vec(), self.init(), tangents=[tg])
         return Ket(out[0], self.n_qubits)
@@ -378,6 +494,10 @@ class PQC:
         """(psi, J) with J[t] = d psi / d theta_t as tensors.  Rows of
         commute=False blocks whose derivative is not a generator sum come
         from a plan of their own each (``special_ops``)."""
+        if self.has_dense():
+            if method not in ("forward", "plain"):
+                raise ValueError(f"unknown method {method!r}")
+            return self._jacobian_dense()
         plan = self.plan(with_generators=True, batch=self.n_true_params)
         specials = self.special_slots()
         if method == "forward":

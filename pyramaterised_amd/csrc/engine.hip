@@ -1352,6 +1352,39 @@ __global__ void k_histogram(const double *__restrict__ v, int64_t n, const doubl
   atomicAdd(counts + idx, 1ull);
 }
 
+// Dense operator on a batch of states: out_b = U x_b (U: dim x dim complex128,
+// row-major).  One wave per output row: lanes stride the row (coalesced) and
+// the wave sums with cross-lane reductions; the row is re-read from L2 for
+// every state of the batch.  Serves gates without a local form (ARBGATE).
+template <typename V>
+__global__ __launch_bounds__(256) void k_apply_dense(const double2 *__restrict__ U, int64_t dim,
+                                                     const V *__restrict__ x, int64_t stride, int64_t batch,
+                                                     V *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (row >= dim) return;
+  const double2 *u = U + row * dim;
+  for (int64_t b = 0; b < batch; ++b) {
+    const V *xb = x + b * stride;
+    double re = 0.0, im = 0.0;
+    for (int64_t k = lane; k < dim; k += 64) {
+      const double2 a = u[k];
+      const V v = xb[k];
+      re += a.x * double(v.x) - a.y * double(v.y);
+      im += a.x * double(v.y) + a.y * double(v.x);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      re += __shfl_xor(re, off);
+      im += __shfl_xor(im, off);
+    }
+    if (lane == 0) {
+      using R = decltype(V{}.x);
+      out[b * stride + row] = V{R(re), R(im)};
+    }
+  }
+}
+
 __global__ void k_abs2(const double2 *__restrict__ z, int64_t n, double *__restrict__ out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) out[i] = z[i].x * z[i].x + z[i].y * z[i].y;
@@ -2446,6 +2479,26 @@ int pqs_histogram(pqs_ctx *ctx, const double *values, int64_t n, const double *e
                        (unsigned long long *)counts);
     HIPCHECK(hipGetLastError());
   }
+  return PQS_OK;
+}
+
+int pqs_apply_dense(pqs_ctx *ctx, int32_t dtype, const double *u, int64_t dim, const void *states,
+                    int64_t stride, int64_t batch, void *out, void *stream) {
+  if (!ctx || !u || !states || !out) return fail(PQS_ERR_ARG, "NULL argument");
+  if (dim <= 0 || batch < 0 || stride < dim) return fail(PQS_ERR_ARG, "bad sizes");
+  if (dim > (int64_t(1) << 14)) return fail(PQS_ERR_UNSUPPORTED, "dense operators are limited to 14 qubits");
+  if (states == out) return fail(PQS_ERR_ARG, "out must not be the input states");
+  if (batch == 0) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned grid = grid_for(dim, 4);
+  if (dtype == PQS_C128)
+    hipLaunchKernelGGL(k_apply_dense<double2>, dim3(grid), dim3(256), 0, st, (const double2 *)u, dim,
+                       (const double2 *)states, stride, batch, (double2 *)out);
+  else
+    hipLaunchKernelGGL(k_apply_dense<float2>, dim3(grid), dim3(256), 0, st, (const double2 *)u, dim,
+                       (const float2 *)states, stride, batch, (float2 *)out);
+  HIPCHECK(hipGetLastError());
   return PQS_OK;
 }
 

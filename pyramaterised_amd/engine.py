@@ -432,6 +432,26 @@ def run(plan, params, init, tangents=None, out=None):
     return out
 
 
+def run_rows(plan, params, states):
+    """Evolve every row of a (B, 2^N) state tensor in place through the plan
+    (row b with params[b], or one shared (P,) vector) -> states."""
+    ctx = context()
+    dev = states.device
+    p = _params_tensor(params, dev, rows=True)
+    if p.numel() == 0:
+        p = torch.zeros((1, 1), dtype=torch.float64, device=dev)
+    B, dim = states.shape
+    if not states.is_contiguous() or dim != 1 << plan.n or states.dtype != CDTYPE[plan.dtype]:
+        raise ValueError("states must be a contiguous (B, 2^N) tensor of the plan's dtype")
+    pstride = p.stride(0) if p.shape[0] > 1 else 0
+    if p.shape[0] not in (1, B):
+        raise ValueError("params must have one row or one row per state")
+    check(L.lib().pqs_run_batch(ctx.handle, plan.handle, ptr(states), dim, B, ptr(p), pstride, None, 0, None,
+                                None, stream_ptr()))
+    _keepalive(p)
+    return states
+
+
 class BatchGraph:
     """All sweep launches of one batched run captured in a HIP graph.
 
@@ -625,6 +645,25 @@ def magic_sum(states, n_qubits, alpha):
     check(L.lib().pqs_magic(ctx.handle, dtype_code(states.dtype), ptr(states), n_qubits, states.shape[0],
                             float(alpha), ptr(out), stream_ptr()))
     return out
+
+
+def apply_dense(U, states):
+    """out_b = U x_b for a dense dim x dim operator (host or device array,
+    complex128) on a (B, dim) or (dim,) state tensor -> new tensor."""
+    ctx = context()
+    squeeze = states.dim() == 1
+    x = states.reshape(1, -1) if squeeze else states
+    x = x.contiguous()
+    B, dim = x.shape
+    u = U if torch.is_tensor(U) else torch.as_tensor(np.ascontiguousarray(U, dtype=np.complex128))
+    u = u.to(device=x.device, dtype=torch.complex128).contiguous()
+    if u.shape != (dim, dim):
+        raise ValueError(f"operator of shape {tuple(u.shape)} on states of dimension {dim}")
+    out = torch.empty_like(x)
+    check(L.lib().pqs_apply_dense(ctx.handle, dtype_code(x.dtype), ptr(u), dim, ptr(x), dim, B, ptr(out),
+                                  stream_ptr()))
+    _keepalive(u, x)
+    return out.reshape(-1) if squeeze else out
 
 
 def abs2(z):

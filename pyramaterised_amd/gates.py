@@ -538,9 +538,55 @@ class ALLTOALL(CHAIN):
         return f"ALL connected {self.entangler.__name__}s"
 
 
+class DenseOperator:
+    """A dense 2^N x 2^N operator (host matrix) with the ``operation``
+    protocol: ``op * ket`` applies it on the GPU (pqs_apply_dense),
+    ``op * op`` multiplies, ``full()`` returns the matrix."""
+
+    def __init__(self, n, matrix):
+        self.q_N = n
+        self.m = np.asarray(matrix, dtype=np.complex128)
+
+    @property
+    def dims(self):
+        return [[2] * self.q_N, [2] * self.q_N]
+
+    def full(self):
+        return self.m.copy()
+
+    def __mul__(self, other):
+        from . import engine as E
+        if isinstance(other, Ket):
+            return Ket(E.apply_dense(self.m, other.t), other.n)
+        if isinstance(other, Gate):
+            other = other.operation
+        if isinstance(other, DenseOperator):
+            return DenseOperator(self.q_N, self.m @ other.m)
+        if isinstance(other, LocalOperator):
+            return DenseOperator(self.q_N, self.m @ other.full())
+        if isinstance(other, (int, float, complex)):
+            return DenseOperator(self.q_N, self.m * other)
+        return NotImplemented
+
+    def __rmul__(self, other):
+        if isinstance(other, (int, float, complex)):
+            return DenseOperator(self.q_N, self.m * other)
+        if isinstance(other, LocalOperator):
+            return DenseOperator(self.q_N, other.full() @ self.m)
+        return NotImplemented
+
+
 class ARBGATE(Gate):
-    """gates.py:407-435: U = expm(-i theta H) for a user operator H.  A dense
-    2^N operator has no local form; the engine does not run it."""
+    """gates.py:407-435: U = expm(-i theta H) for a user operator H on all
+    q_N qubits, and the reference's derivative D = -i H / 2 (half of
+    dU/dtheta = -i H U; kept as written), so the derivative state applies
+    D U.  A dense 2^N operator has no local-gate form: PQC runs a circuit
+    that holds one in segments of local gates, with U (and D U) built on the
+    host -- from an eigen-decomposition of H, computed once, when H is
+    Hermitian, else scipy's expm -- and applied on the GPU in between
+    (pqs_apply_dense, up to 14 qubits)."""
+
+    dense = True
 
     def __init__(self, Ham):
         self._Ham = Ham
@@ -548,10 +594,47 @@ class ARBGATE(Gate):
         self.phi = 0
         self.is_param = True
         self.param_count = 1
-        self.q_N = len(Ham.dims[0]) if hasattr(Ham, "dims") else int(np.log2(len(Ham)))
+        if hasattr(Ham, "full"):
+            H = np.asarray(Ham.full(), dtype=np.complex128)
+        else:
+            H = np.asarray(Ham, dtype=np.complex128)
+        if H.ndim != 2 or H.shape[0] != H.shape[1] or H.shape[0] & (H.shape[0] - 1):
+            raise ValueError("ARBGATE needs a square 2^N x 2^N operator")
+        self._H = H
+        self.q_N = int(H.shape[0]).bit_length() - 1
+        self._eig = None
 
     def prim_ops(self, slot):
-        raise NotImplementedError("ARBGATE (a dense 2^N generator) is not supported by the GPU engine")
+        raise NotImplementedError("ARBGATE is a dense operator: PQC applies it between plans")
+
+    def gen_ops(self, tangent, slot, which=1):
+        raise NotImplementedError("ARBGATE is a dense operator: PQC applies its derivative between plans")
+
+    def matrix(self, theta=None):
+        """U = expm(-i theta H) (gates.py:416-419)."""
+        import scipy.linalg
+        th = float(self.theta if theta is None else theta)
+        if self._eig is None:
+            herm = np.allclose(self._H, self._H.conj().T, atol=1e-14, rtol=0)
+            self._eig = np.linalg.eigh(self._H) if herm else False
+        if self._eig is not False:
+            w, V = self._eig
+            return (V * np.exp(-1j * th * w)) @ V.conj().T
+        return scipy.linalg.expm(-1j * th * self._H)
+
+    def derivative_matrix(self, theta=None):
+        """D U with the reference's D = -i H / 2 (gates.py:427-429)."""
+        return (-0.5j * self._H) @ self.matrix(theta)
+
+    @property
+    def operation(self):
+        return DenseOperator(self.q_N, self.matrix())
+
+    def get_op(self):
+        return self.operation
+
+    def derivative(self):
+        return DenseOperator(self.q_N, -0.5j * self._H)
 
     def set_theta(self, theta):
         self.theta = theta

@@ -313,7 +313,8 @@ class Measurements:
         psi = self.QC.state
         psi_t, J = self.QC.jacobian()
         self.gradient_list = [Ket(J[i], self.QC.n_qubits) for i in range(J.shape[0])]
-        Hpsi = self.QC.H * psi
+        H = self.QC.hamiltonian() if hasattr(self.QC, "hamiltonian") else self.QC.H
+        Hpsi = H * psi
         g = E.gram(Hpsi.t.reshape(1, -1), J)[0]
         return list((2 * g.real).cpu().numpy())
 

@@ -599,21 +599,32 @@ class PauliSum:
 
     @staticmethod
     def from_dense(op):
-        """Pauli decomposition of a small dense operator (n <= 10)."""
+        """Pauli decomposition of a dense operator (n <= 12), in O(n 4^n):
+        with P = X^x Z^z, Tr(P^dag A) = sum_j (-1)^{z.j} A[j^x, j], so for
+        each X mask x the coefficients over all Z masks are one
+        Walsh-Hadamard transform of A's x-th off-diagonal; a Y on a qubit is
+        i X Z there."""
         a = np.asarray(op.full() if hasattr(op, "full") else op, dtype=np.complex128)
         n = _qubits_of(a.shape[0])
-        if n > 10:
-            raise ValueError("dense operators above 10 qubits: pass a PauliSum")
-        out = {}
-        import itertools
-        for s in itertools.product(range(4), repeat=n):
-            k = np.array([[1.0 + 0j]])
-            for code in s:
-                k = np.kron(k, _PMAT[code])
-            c = np.trace(k.conj().T @ a) / 2 ** n
-            if abs(c) > 1e-15:
-                out[s] = c
-        return PauliSum(n, out)
+        if n > 12:
+            raise ValueError("dense operators above 12 qubits: pass a PauliSum")
+        d = 2 ** n
+        j = np.arange(d)
+        m = a[j[None, :] ^ j[:, None], j[None, :]]  # m[x, j] = A[j ^ x, j]
+        h = 1
+        while h < d:  # Walsh-Hadamard transform along j
+            m = m.reshape(d, -1, 2, h)
+            m = np.stack([m[:, :, 0, :] + m[:, :, 1, :], m[:, :, 0, :] - m[:, :, 1, :]], axis=2)
+            h *= 2
+        m = m.reshape(d, d) / d  # m[x, z] = Tr((X^x Z^z)^dag A) / d
+        xs, zs = np.nonzero(np.abs(m) > 1e-15)
+        shifts = (n - 1 - np.arange(n))[None, :]
+        bx, bz = (xs[:, None] >> shifts) & 1, (zs[:, None] >> shifts) & 1
+        codes = np.where(bz == 0, bx, np.where(bx == 0, 3, 2))  # I=0 X=1 Y=2 Z=3
+        ny = np.sum(bx & bz, axis=1)
+        # a Y-string is i^ny X^x Z^z, so its coefficient is (-i)^ny m[x, z]
+        coef = m[xs, zs] * (-1j) ** ny
+        return PauliSum(n, dict(zip(map(tuple, codes.tolist()), coef.tolist())))
 
     def __repr__(self):
         return f"PauliSum(n={self.n}, terms={len(self.d)})"

@@ -353,3 +353,26 @@ def test_identity_gate_and_new_measure_closed_forms():
     F = Q @ np.diag(ev) @ Q.T
     assert abs(M.new_measure(F) - 2.25) < 1e-12
     assert abs(M.new_measure(np.eye(5)) - 5) < 1e-12
+
+
+def test_arbgate_restatement_pins():
+    """ARBGATE (gates.py:407-435) restated with scipy's expm: for a Pauli
+    string P, expm(-i t P) = cos t I - i sin t P (note: no half angle, unlike
+    the rotations), and the derivative state is -i/2 H U psi, half of the
+    central difference of the state (the reference's factor)."""
+    n = 3
+    P = np.kron(np.kron(OG.SX, OG.SZ), OG.SY)
+    g = C.ARBGATE(P, n)
+    t = 0.73
+    (qs, U), = g.prims(t)
+    assert qs == (0, 1, 2)
+    assert np.allclose(U, np.cos(t) * np.eye(8) - 1j * np.sin(t) * P, atol=1e-14)
+    rng = np.random.default_rng(2)
+    A = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    H = A + A.conj().T
+    gates = [C.R_y(0, n), C.ARBGATE(H, n), C.R_x(2, n)]
+    th = np.array([0.4, 0.9, 1.7])
+    d = S.derivative_states(gates, th, n)
+    h = 1e-6
+    fd = (S.run(gates, th + [0, h, 0], n) - S.run(gates, th - [0, h, 0], n)) / (2 * h)
+    assert np.max(np.abs(d[1] - 0.5 * fd)) < 1e-8
