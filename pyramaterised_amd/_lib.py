@@ -86,6 +86,20 @@ class EngineError(RuntimeError):
     pass
 
 
+_fingerprint = None
+
+
+def library_fingerprint():
+    """Hash of the built library: part of the tile-shape tuner's cache key, so
+    a rebuilt planner or kernel set is tuned afresh."""
+    global _fingerprint
+    if _fingerprint is None:
+        import hashlib
+        with open(LIB_PATH, "rb") as f:
+            _fingerprint = hashlib.sha1(f.read()).hexdigest()[:16]
+    return _fingerprint
+
+
 def lib():
     """The loaded library; raises if it has not been built."""
     global _lib

@@ -1620,8 +1620,12 @@ int jit_code_cached(const std::string &src, std::vector<char> &code, std::string
       return PQS_OK;
     }
   }
-  uint64_t h = 1469598103934665603ull;  // FNV-1a of the source
+  // FNV-1a of the source and of what else decides the code object (compile
+  // options, hiprtc version)
+  static const std::string sig = pqs::jit_signature();
+  uint64_t h = 1469598103934665603ull;
   for (unsigned char c : src) h = (h ^ c) * 1099511628211ull;
+  for (unsigned char c : sig) h = (h ^ c) * 1099511628211ull;
   const char *env = std::getenv("PQS_JIT_CACHE");
   const char *tmp = std::getenv("TMPDIR");
   std::string dir = env ? env : (std::string(tmp ? tmp : "/tmp") + "/pqs_jit_cache");

@@ -587,15 +587,30 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
   }
 }
 
+static const char *kJitOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                 "-Xclang", "-packed-fp32-ops"};
+
+// What besides the source decides the code object: the compile options and
+// the hiprtc version (part of the on-disk cache key).
+std::string jit_signature() {
+  std::string sig;
+  for (const char *o : kJitOpts) {
+    sig += o;
+    sig += ' ';
+  }
+  int major = 0, minor = 0;
+  hiprtcVersion(&major, &minor);
+  sig += "hiprtc " + std::to_string(major) + "." + std::to_string(minor);
+  return sig;
+}
+
 int jit_compile(const std::string &src, std::vector<char> &code, std::string &log) {
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "pqs_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
     log = "hiprtcCreateProgram failed";
     return PQS_ERR_HIP;
   }
-  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
-                        "-Xclang", "-packed-fp32-ops"};
-  hiprtcResult r = hiprtcCompileProgram(prog, 7, opts);
+  hiprtcResult r = hiprtcCompileProgram(prog, int(sizeof kJitOpts / sizeof kJitOpts[0]), kJitOpts);
   size_t ls = 0;
   hiprtcGetProgramLogSize(prog, &ls);
   log.assign(ls, '\0');

@@ -146,4 +146,5 @@ std::string jit_source(const pqs_plan &p, std::vector<int> &sweeps);
 std::string jit_source_sweep(const pqs_plan &p, int s);
 bool jit_sweep_ok(const pqs_plan &p, int s);
 int jit_compile(const std::string &src, std::vector<char> &code, std::string &log);
+std::string jit_signature();  // compile options and hiprtc version
 }

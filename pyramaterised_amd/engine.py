@@ -180,7 +180,8 @@ def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
     cands = tile_candidates(n_qubits, dtype_c)
     arr = (L.Op * max(1, len(ops)))(*ops)
     B = 1 if n_qubits >= 26 else max(1, min(int(batch), (1 << 26) >> n_qubits))
-    key = hashlib.sha1(bytes(arr) + repr((n_qubits, dtype_c, B, cands)).encode()).hexdigest()[:24]
+    key = hashlib.sha1(bytes(arr) + repr((n_qubits, dtype_c, B, cands)).encode() +
+                       L.library_fingerprint().encode()).hexdigest()[:24]
     choice = _tune_memo.get(key)
     path = os.path.join(jit_cache_dir(), f"tune_{key}.json")
     if choice is None and os.path.exists(path):

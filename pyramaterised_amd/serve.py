@@ -256,7 +256,9 @@ class Server:
         self.circuits = {}
         self.queues = OrderedDict()  # class key -> deque of requests (FIFO)
         self.results = {}
-        self._held = {}  # rid -> bytes held by a resident state
+        self._held = {}    # rid -> group of the tensor holding its resident state
+        self._groups = {}  # group -> [rows not yet released, bytes held]
+        self._next_group = 0
         self.n_batches = 0
         self.batch_sizes = []
         self.peak_in_use = 0
@@ -310,11 +312,20 @@ class Server:
         return self._n_inflight
 
     def release(self, rid):
-        """Drop a resident state and return its bytes to the budget."""
+        """Drop a resident state.  States of one batched launch are rows of
+        one device tensor, which is freed only when its last row is
+        released, so their bytes return to the budget together then."""
         r = self.results.get(rid)
         if r is not None:
             r.state = None
-        self.in_use -= self._held.pop(rid, 0)
+        h = self._held.pop(rid, None)
+        if h is None:
+            return
+        group = self._groups[h]
+        group[0] -= 1
+        if group[0] == 0:
+            self.in_use -= group[1]
+            del self._groups[h]
 
     # ---------------------------------------------------------------- loop
     def _free_slot(self, large):
@@ -419,6 +430,9 @@ class Server:
                 b = slot.inflight.popleft()
                 self._n_inflight -= 1
                 kind = b.reqs[0].kind
+                group = self._next_group
+                self._next_group += 1
+                held_total = 0
                 if kind == STATE_MW:
                     qs = b.q.cpu().numpy()
                     for i, r in enumerate(b.reqs):
@@ -428,7 +442,8 @@ class Server:
                         if self.keep_states:
                             res.state = b.out[i]
                             held = self.amp * (1 << r.n)
-                            self._held[r.rid] = held
+                            self._held[r.rid] = group
+                            held_total += held
                             self.in_use -= self.request_bytes(r) - held
                         else:
                             self.in_use -= self.request_bytes(r)
@@ -440,10 +455,13 @@ class Server:
                     if self.keep_states:
                         res.state = b.out
                         held = self.amp * (1 << r.n)
-                        self._held[r.rid] = held
+                        self._held[r.rid] = group
+                        held_total += held
                         self.in_use -= self.request_bytes(r) - held
                     else:
                         self.in_use -= self.request_bytes(r)
+                if held_total:
+                    self._groups[group] = [len(b.reqs), held_total]
                 done += len(b.reqs)
         return done
 

@@ -85,6 +85,25 @@ def test_release_returns_budget(sv):
     assert srv.in_use == 0
 
 
+def test_batched_states_release_together(sv):
+    """States of one batched launch are rows of one tensor: their bytes stay
+    counted until the last row of the batch is released."""
+    import numpy as np
+    reqs = [sv.Request(i, 7, 3, sv.STATE_MW, np.random.default_rng(i).random(42) * 6.28) for i in range(3)]
+    srv = sv.Server(n_streams=1)
+    res = srv.run(reqs, realtime=False)
+    assert srv.batch_sizes == [3]
+    full = srv.in_use
+    assert full == 3 * 16 * (1 << 7)
+    srv.release(res[0].rid)
+    srv.release(res[1].rid)
+    assert srv.in_use == full
+    srv.release(res[2].rid)
+    assert srv.in_use == 0
+    srv.release(res[2].rid)  # releasing twice is a no-op
+    assert srv.in_use == 0
+
+
 def test_specialised_class_matches_alone(sv):
     """Classes large enough for run-time specialised kernels: after the
     background compile is loaded, results equal the interpreted path."""
