@@ -286,6 +286,51 @@ def speedups_16q(pyqc, args, head):
             "qfim16_plain_resim_s": t_resim, "qfim16_overlap_call_us": t_ov * 1e6, "jac_forward_vs_plain_maxdiff": jdiff}
 
 
+def leg_adjoint20(pyqc):
+    """Energy gradient of a 20-qubit, 40-layer HEA (P = 1600, H = Z0 Z1):
+    the adjoint pass (two states) against the forward-Jacobian route (P
+    derivative states + one-row Gram); time, device memory high-water and
+    the largest difference between the two gradients."""
+    import torch
+    from pyramaterised_amd import adjoint
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    n, p = 20, 40
+    qc = pyqc.PQC(n)
+    for l in T.clifford_HE_layers(p, n):
+        qc.add_layer(l)
+    P = qc.n_true_params
+    th = np.random.default_rng(20).random(P) * 2 * np.pi
+    qc.set_params(list(th))
+    H = qc.hamiltonian()
+
+    def jac_route():
+        psi, J = qc.jacobian()
+        Hpsi = adjoint.apply_hamiltonian(H, psi, n, adjoint=True)
+        g = E.gram(Hpsi.reshape(1, -1), J)[0]
+        return (2 * g.real).cpu().numpy()
+
+    out = {"n_qubits": n, "layers": p, "params": P}
+    for name, f in (("adjoint", lambda: adjoint.energy_and_gradient(qc, th)[1]), ("jacobian", jac_route)):
+        f()  # untimed: plans, adjoint program
+        sync()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            g = f()
+        sync()
+        out[f"{name}_ms"] = (time.perf_counter() - t0) / reps * 1e3
+        out[f"{name}_peak_GB"] = (torch.cuda.max_memory_allocated() - base) / 1e9
+        out[f"{name}_grad"] = g
+        torch.cuda.empty_cache()
+    out["max_abs_diff"] = float(np.max(np.abs(out.pop("adjoint_grad") - out.pop("jacobian_grad"))))
+    out["speedup"] = out["jacobian_ms"] / out["adjoint_ms"]
+    out["measurement_points"] = len(adjoint.program(qc).steps)
+    return out
+
+
 def leg_serving(args):
     """Serving mix: Poisson requests (HEA, 6-20 qubits, depth 2-40, 70% state
     + Meyer-Wallach, 30% QFIM, complex128) through the continuous-batching
@@ -374,8 +419,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-rate", type=float, default=25.0)
     ap.add_argument("--quick", action="store_true", help="headline only")
-    ap.add_argument("--legs", default="28q,qfim12,serving,speedups",
-                    help="extra legs after the headline (comma list of 28q, qfim12, serving, speedups)")
+    ap.add_argument("--legs", default="28q,qfim12,adjoint20,serving,speedups",
+                    help="extra legs after the headline (comma list of 28q, qfim12, adjoint20, serving, speedups)")
     ap.add_argument("--only-legs", action="store_true",
                     help="skip the headline and the CPU baseline; print the legs' JSON only (for profiling a leg)")
     ap.add_argument("--shape", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
@@ -418,6 +463,10 @@ def main():
             log("12-qubit TFIM QFIM")
             extra["qfim_12q_tfim"] = leg_qfim12(pyqc)
             log(json.dumps(extra["qfim_12q_tfim"]))
+        if "adjoint20" in legs:
+            log("20-qubit energy gradient: adjoint pass vs forward Jacobian")
+            extra["gradient_20q_adjoint"] = leg_adjoint20(pyqc)
+            log(json.dumps(extra["gradient_20q_adjoint"]))
         if "serving" in legs:
             log(f"serving mix: {args.serve_requests} requests at {args.serve_rate}/s")
             extra["serving"] = leg_serving(args)

@@ -296,6 +296,23 @@ int pqs_expect_pauli(pqs_ctx *ctx, int32_t dtype, const void *states_dev, int32_
                      int64_t batch, const uint64_t *xmask_host, const uint64_t *zmask_host,
                      const double *coef_host, int32_t n_terms, double *out_dev, void *stream);
 
+/* out = H x for H = sum_t coef_t P_t (Pauli strings as in pqs_expect_pauli)
+ * on one state (out must not alias x).  The lambda = H psi that starts the
+ * adjoint gradient's backward pass (reference measure.py:461-471, which
+ * forms H |d psi> per parameter). */
+int pqs_apply_pauli(pqs_ctx *ctx, int32_t dtype, const void *x_dev, int32_t n_qubits, const uint64_t *xmask_host,
+                    const uint64_t *zmask_host, const double *coef_host, int32_t n_terms, void *out_dev,
+                    void *stream);
+
+/* out[t] = <bra| coef_t P_t |ket> (complex128, (re, im)) for every term t
+ * (Pauli strings as in pqs_expect_pauli), two-stage sums in a fixed order.
+ * With bra = lambda and ket = psi at a point of the adjoint backward pass,
+ * 2 Re of a parameter's terms is its energy gradient 2 Re <psi|H|d psi>
+ * (reference Measurements.get_gradient_vector, measure.py:461-471). */
+int pqs_pauli_transition(pqs_ctx *ctx, int32_t dtype, const void *bra_dev, const void *ket_dev, int32_t n_qubits,
+                         const uint64_t *xmask_host, const uint64_t *zmask_host, const double *coef_host,
+                         int32_t n_terms, double *out_dev, void *stream);
+
 /* Histogram of values into n_bins int64 counts over the n_bins + 1 ascending
  * edges (device array, e.g. numpy.linspace(0, 1, n_bins + 1)) with
  * numpy.histogram's rules: values outside [edges[0], edges[n_bins]] are

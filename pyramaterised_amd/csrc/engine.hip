@@ -1335,6 +1335,63 @@ __global__ __launch_bounds__(RB) void k_expect_partial(const typename CT<R>::V *
   if (threadIdx.x == 0) part[blockIdx.x] = double2{v[0], v[1]};
 }
 
+// out = sum_t c_t P_t x (Pauli strings as index masks; c_t includes i^{#Y}):
+// (P x)[j] = (-1)^{popc((j ^ xm) & zm)} x[j ^ xm].  The Hamiltonian applied
+// to the state at the start of the adjoint gradient's backward pass.
+template <typename V>
+__global__ __launch_bounds__(256) void k_apply_pauli(const V *__restrict__ x, int64_t dim,
+                                                     const PauliTerm *__restrict__ terms, int nt,
+                                                     V *__restrict__ out) {
+  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= dim) return;
+  double re = 0.0, im = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    const uint64_t jj = uint64_t(j) ^ terms[t].x;
+    const V y = x[jj];
+    double yr = double(y.x), yi = double(y.y);
+    if (__popcll(jj & terms[t].z) & 1) {
+      yr = -yr;
+      yi = -yi;
+    }
+    re += terms[t].cre * yr - terms[t].cim * yi;
+    im += terms[t].cre * yi + terms[t].cim * yr;
+  }
+  using R = decltype(V{}.x);
+  out[j] = V{R(re), R(im)};
+}
+
+// <bra| c_t P_t |ket> for every term t: block (chunk, t) sums one chunk of
+// one term; k_sum_rows reduces the chunks (fixed order).  The gradient
+// terms of the adjoint backward pass, 2 Re <lambda| D_p |psi> per parameter.
+template <typename V>
+__global__ __launch_bounds__(RB) void k_transition_partial(const V *__restrict__ bra, const V *__restrict__ ket,
+                                                           int64_t dim, int64_t n_chunks,
+                                                           const PauliTerm *__restrict__ terms,
+                                                           double2 *__restrict__ part) {
+  __shared__ double sh[2 * RB];
+  const int64_t t = blockIdx.y;
+  const int64_t chunk = blockIdx.x;
+  const PauliTerm pt = terms[t];
+  const int64_t lo = chunk * kInnerChunk;
+  const int64_t hi = lo + kInnerChunk < dim ? lo + kInnerChunk : dim;
+  double v[2] = {0, 0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += RB) {
+    const V a = bra[i];
+    const uint64_t ii = uint64_t(i) ^ pt.x;
+    const V y = ket[ii];
+    double re = double(a.x) * y.x + double(a.y) * y.y;  // conj(a) * y
+    double im = double(a.x) * y.y - double(a.y) * y.x;
+    if (__popcll(ii & pt.z) & 1) {
+      re = -re;
+      im = -im;
+    }
+    v[0] += pt.cre * re - pt.cim * im;
+    v[1] += pt.cre * im + pt.cim * re;
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) part[t * n_chunks + chunk] = double2{v[0], v[1]};
+}
+
 __global__ void k_histogram(const double *__restrict__ v, int64_t n, const double *__restrict__ edges, int nb,
                             unsigned long long *__restrict__ counts) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -2417,6 +2474,89 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
     HIPCHECK(hipGetLastError());
     return PQS_OK;
   }
+}
+
+// Pauli terms from qubit masks (bit q = qubit q; Y = X and Z) to index masks
+// (bit n-1-q), with i^{#Y} folded into the coefficient.
+static std::vector<PauliTerm> pauli_terms(int n, const uint64_t *xm, const uint64_t *zm, const double *coef,
+                                          int nt) {
+  std::vector<PauliTerm> h(size_t(std::max(nt, 1)));
+  for (int t = 0; t < nt; ++t) {
+    uint64_t x = 0, z = 0;
+    int ny = 0;
+    for (int q = 0; q < n; ++q) {
+      const bool bx = (xm[t] >> q) & 1, bz = (zm[t] >> q) & 1;
+      if (bx) x |= uint64_t(1) << (n - 1 - q);
+      if (bz) z |= uint64_t(1) << (n - 1 - q);
+      if (bx && bz) ++ny;
+    }
+    double cr = coef[2 * t], ci = coef[2 * t + 1];
+    for (int r = 0; r < ny; ++r) {
+      const double t2 = cr;
+      cr = -ci;
+      ci = t2;
+    }
+    h[size_t(t)] = PauliTerm{x, z, cr, ci};
+  }
+  return h;
+}
+
+int pqs_apply_pauli(pqs_ctx *ctx, int32_t dtype, const void *x, int32_t n, const uint64_t *xm, const uint64_t *zm,
+                    const double *coef, int32_t nt, void *out, void *stream) {
+  if (!ctx || !x || !out || (nt > 0 && (!xm || !zm || !coef))) return fail(PQS_ERR_ARG, "NULL argument");
+  if (n < 1 || n > pqs::kMaxQubits || nt < 0) return fail(PQS_ERR_ARG, "bad sizes");
+  if (x == out) return fail(PQS_ERR_ARG, "out must not be the input state");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t dim = int64_t(1) << n;
+  std::vector<PauliTerm> h = pauli_terms(n, xm, zm, coef, nt);
+  int rc = ws_reserve(ctx, h.size() * sizeof(PauliTerm), st);
+  if (rc) return rc;
+  PauliTerm *d_terms = (PauliTerm *)ctx->ws;
+  rc = upload(ctx, d_terms, h.data(), h.size() * sizeof(PauliTerm), st);
+  if (rc) return rc;
+  if (dtype == PQS_C128)
+    hipLaunchKernelGGL(k_apply_pauli<double2>, dim3(grid_for(dim, 256)), dim3(256), 0, st, (const double2 *)x, dim,
+                       d_terms, nt, (double2 *)out);
+  else
+    hipLaunchKernelGGL(k_apply_pauli<float2>, dim3(grid_for(dim, 256)), dim3(256), 0, st, (const float2 *)x, dim,
+                       d_terms, nt, (float2 *)out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
+int pqs_pauli_transition(pqs_ctx *ctx, int32_t dtype, const void *bra, const void *ket, int32_t n,
+                         const uint64_t *xm, const uint64_t *zm, const double *coef, int32_t nt, double *out,
+                         void *stream) {
+  if (!ctx || !bra || !ket || !out || (nt > 0 && (!xm || !zm || !coef))) return fail(PQS_ERR_ARG, "NULL argument");
+  if (n < 1 || n > pqs::kMaxQubits || nt < 0) return fail(PQS_ERR_ARG, "bad sizes");
+  if (nt == 0) return PQS_OK;
+  if (nt > 65535) return fail(PQS_ERR_UNSUPPORTED, "too many terms");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t dim = int64_t(1) << n;
+  const int64_t nch = (dim + kInnerChunk - 1) / kInnerChunk;
+  std::vector<PauliTerm> h = pauli_terms(n, xm, zm, coef, nt);
+  const size_t part_b = size_t(nt * nch) * sizeof(double2);
+  const size_t off_t = (part_b + 255) & ~size_t(255);
+  int rc = ws_reserve(ctx, off_t + h.size() * sizeof(PauliTerm), st);
+  if (rc) return rc;
+  char *ws = (char *)ctx->ws;
+  PauliTerm *d_terms = (PauliTerm *)(ws + off_t);
+  rc = upload(ctx, d_terms, h.data(), h.size() * sizeof(PauliTerm), st);
+  if (rc) return rc;
+  if (nch > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "state too large");
+  const dim3 grid{unsigned(nch), unsigned(nt), 1};
+  if (dtype == PQS_C128)
+    hipLaunchKernelGGL(k_transition_partial<double2>, grid, dim3(RB), 0, st, (const double2 *)bra,
+                       (const double2 *)ket, dim, nch, d_terms, (double2 *)ws);
+  else
+    hipLaunchKernelGGL(k_transition_partial<float2>, grid, dim3(RB), 0, st, (const float2 *)bra,
+                       (const float2 *)ket, dim, nch, d_terms, (double2 *)ws);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_rows, dim3(unsigned(nt)), dim3(RB), 0, st, (const double2 *)ws, nch, (double2 *)out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
 }
 
 int pqs_expect_pauli(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t batch, const uint64_t *xm,

@@ -308,15 +308,42 @@ class Measurements:
 
     # ------------------------------------------------------------- training
     def get_gradient_vector(self, theta):
-        """measure.py:461-471: dE/dtheta_i = 2 Re <psi|H|d_i psi>."""
+        """measure.py:461-471: dE/dtheta_i = 2 Re <psi|H|d_i psi>, by the
+        adjoint pass (two states, one backward sweep of small plans) where
+        the circuit allows it, else from the forward Jacobian.  The
+        derivative states (``gradient_list``, used by QNG) are built on first
+        access."""
         self.QC.state = self.QC.run(angles=theta)
+        self._grad_states = None
+        try:
+            from . import adjoint
+            _, g = adjoint.energy_and_gradient(self.QC, self.QC._params_vec())
+            return list(g)
+        except NotImplementedError:
+            pass
         psi = self.QC.state
-        psi_t, J = self.QC.jacobian()
-        self.gradient_list = [Ket(J[i], self.QC.n_qubits) for i in range(J.shape[0])]
+        J = self._jacobian_states()
         H = self.QC.hamiltonian() if hasattr(self.QC, "hamiltonian") else self.QC.H
         Hpsi = H * psi
         g = E.gram(Hpsi.t.reshape(1, -1), J)[0]
         return list((2 * g.real).cpu().numpy())
+
+    def _jacobian_states(self):
+        _, J = self.QC.jacobian()
+        self._grad_states = [Ket(J[i], self.QC.n_qubits) for i in range(J.shape[0])]
+        return J
+
+    @property
+    def gradient_list(self):
+        """The derivative states at the circuit's current parameters
+        (measure.py:468, where get_gradient_vector stores them)."""
+        if getattr(self, "_grad_states", None) is None:
+            self._jacobian_states()
+        return self._grad_states
+
+    @gradient_list.setter
+    def gradient_list(self, states):
+        self._grad_states = states
 
     def train(self, epsilon=1e-6, rate=0.001, method="gradient", angles="random", verbose=False):
         """measure.py:473-553."""

@@ -678,3 +678,62 @@ def test_generate_circuit_types_match_oracle_templates(pyqc, case):
     assert np.max(np.abs(a - OS.run(og, th, n, init=init))) < 1e-12
     if start == "half":
         assert np.allclose(np.asarray(qc.initial_state.full()).ravel(), init)
+
+
+def _adjoint_cpu(prog, psi, lam, params, n):
+    """The adjoint backward pass with the program's op lists on the CPU."""
+    grad = np.zeros(prog.P)
+    for ops, group in prog.step_ops:
+        lam = simulate_ops(ops, range(len(ops)), n, params, init=lam)
+        psi = simulate_ops(ops, range(len(ops)), n, params, init=psi)
+        for slot, terms in group:
+            for c, x, z in terms:
+                Pm = np.array([[1.0 + 0j]])
+                for q in range(n):
+                    bx, bz = (x >> q) & 1, (z >> q) & 1
+                    Pm = np.kron(Pm, OG.I2 if not (bx or bz) else OG.SX if not bz else OG.SZ if not bx else OG.SY)
+                grad[slot] += 2 * np.real(c * np.vdot(lam, Pm @ psi))
+    return grad
+
+
+@pytest.mark.parametrize("which", ["clifford", "tfim", "random"])
+def test_adjoint_program_matches_oracle_gradient(pyqc, which):
+    """The adjoint program's steps (inverse ops between measurement points,
+    grouped generator terms), run on the CPU, give the oracle's energy
+    gradient 2 Re <H psi|d_p psi> for H = Z0 Z1 (measure.py:461-471); an
+    HEA layer needs two measurement points."""
+    from pyramaterised_amd import adjoint
+    from pyramaterised_amd import templates as T
+    n = 5
+    rng = np.random.default_rng(9)
+    if which == "clifford":
+        layers, og = T.clifford_HE_layers(3, n), OC.clifford_he(3, n)
+    elif which == "tfim":
+        layers, og = T.TFIM_layers(2, n), OC.tfim(2, n)
+    else:
+        spec = [s for s in random_spec(n, 3, rng, fsim=False) if s[0] != "sqrtH"]
+        layers, og = [build_pkg(spec, n, pyqc)], build_oracle(spec, n)
+    qc = pyqc.PQC(n)
+    for l in layers:
+        qc.add_layer(l)
+    prog = adjoint.AdjointProgram(qc)
+    if which == "clifford":
+        assert len(prog.step_ops) == 2 * 3
+    th = rng.random(qc.n_true_params) * 2 * np.pi
+    psi = OS.run(og, th, n)
+    zz = np.where(((np.arange(2 ** n) >> (n - 1)) ^ (np.arange(2 ** n) >> (n - 2))) & 1, -1.0, 1.0)
+    g = _adjoint_cpu(prog, psi, zz * psi, th, n)
+    d = OS.derivative_states(og, th, n)
+    want = np.array([2 * np.real(np.vdot(zz * psi, di)) for di in d])
+    assert np.max(np.abs(g - want)) < 1e-12
+
+
+def test_adjoint_program_refuses_non_unitary_gates(pyqc):
+    """sqrtH (element-wise square root of H, gates.py:235-242) is not
+    unitary: the adjoint pass cannot uncompute it, so it declines (and the
+    callers take the Jacobian route)."""
+    from pyramaterised_amd import adjoint
+    qc = pyqc.PQC(2)
+    qc.add_layer([pyqc.R_x(0, 2), pyqc.sqrtH(0, 2), pyqc.R_y(1, 2)])
+    with pytest.raises(NotImplementedError):
+        adjoint.AdjointProgram(qc)
