@@ -207,6 +207,23 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *plan, void *states_dev, int64_t state_
                   const int32_t *tangents_host, int32_t init, const void *src_dev,
                   const void *init_table_dev, void *stream);
 
+/* Instances of several plans in one run: instance i evolves with plan
+ * plans[inst_plan[i]] (fields as in pqs_run; init 2 takes the per-instance
+ * product table init_tabs[i], init_tabs may be NULL otherwise).  The plans
+ * may differ in width and depth but share one dtype.  Sweep round r is one
+ * launch per group of plans with the same tile size (register-phase plans;
+ * all plans with tiles below 10 qubits form one group) over sweep r of every
+ * instance whose plan has more than r sweeps: its blocks are the
+ * instances' tiles laid end to end, each block finding its instance by a
+ * binary search.  Kernels are the interpreted sweep executors (run-time
+ * specialised kernels are per plan).  Tangent instances are accepted for
+ * plans without split generators.  No reference counterpart: it serves
+ * callers that batch circuits of uneven width and depth (a serving loop's
+ * width classes, the reference's sampling at measure.py:132 over many
+ * circuits). */
+int pqs_run_multi(pqs_ctx *ctx, pqs_plan *const *plans, int32_t n_plans, const pqs_instance *insts,
+                  const int32_t *inst_plan, const void *const *init_tabs, int64_t n_inst, void *stream);
+
 /* Graph-capturable batched runs.  pqs_batch_table writes the instance table
  * of a strided batch (arguments as pqs_run_batch; init 0 needs src_dev) into
  * the caller's device buffer table_dev (batch * pqs_batch_table_bytes(1)

@@ -64,6 +64,7 @@ SIGNATURES = {
     "pqs_apply_dense": (C.c_int, [vp, i32, vp, i64, vp, i64, i64, vp, vp]),
     "pqs_apply_pauli": (C.c_int, [vp, i32, vp, i32, vp, vp, vp, i32, vp, vp]),
     "pqs_pauli_transition": (C.c_int, [vp, i32, vp, vp, i32, vp, vp, vp, i32, vp, vp]),
+    "pqs_run_multi": (C.c_int, [vp, vp, i32, vp, vp, vp, i64, vp]),
     "pqs_run_batch_noalloc": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, i32, vp, vp, vp, vp]),
     "pqs_jacobian": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, vp]),
     "pqs_inner": (C.c_int, [vp, i32, vp, i64, vp, vp, i64, vp, vp]),

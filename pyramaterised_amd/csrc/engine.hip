@@ -319,21 +319,18 @@ __device__ __forceinline__ void apply_lds_op(typename CT<R>::V *tile, const int 
   }
 }
 
+// One tile of one instance through one sweep, op by op in LDS (tiles below
+// 10 qubits).  `sw` is the sweep (a kernel argument, or a record in device
+// memory for launches that mix plans).
 template <typename R, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_sweep(const DevSweep sw, const DevOp *__restrict__ ops,
-                                                 const DevTerm *__restrict__ terms,
-                                                 const InstDev *__restrict__ insts, int64_t n_active,
-                                                 const typename CT<R>::V *__restrict__ init_tab, int n,
-                                                 int s, int n_sweeps) {
+__device__ __forceinline__ void sweep_lds_body(const DevSweep &sw, const DevOp *__restrict__ ops,
+                                               const DevTerm *__restrict__ terms, const InstDev &in,
+                                               const int64_t tile_id, const typename CT<R>::V *__restrict__ init_tab,
+                                               int n, int s, int n_sweeps) {
   using V = typename CT<R>::V;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V *tile = reinterpret_cast<V *>(smem_raw);
   const int tid = threadIdx.x;
-  const int64_t gid = blockIdx.x;
-  const int64_t inst_idx = gid / sw.n_tiles;
-  const int64_t tile_id = gid - inst_idx * sw.n_tiles;
-  if (inst_idx >= n_active) return;
-  const InstDev in = insts[inst_idx];
   const int S = 1 << sw.k;
 
   uint64_t base = 0;
@@ -396,6 +393,20 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const DevSweep sw, const DevOp 
       gi |= uint64_t((j >> sw.run_shift[r]) & ((1 << sw.run_len[r]) - 1)) << sw.run_pos[r];
     dst[gi] = tile[j];
   }
+}
+
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sweep(const DevSweep sw, const DevOp *__restrict__ ops,
+                                                 const DevTerm *__restrict__ terms,
+                                                 const InstDev *__restrict__ insts, int64_t n_active,
+                                                 const typename CT<R>::V *__restrict__ init_tab, int n,
+                                                 int s, int n_sweeps) {
+  const int64_t gid = blockIdx.x;
+  const int64_t inst_idx = gid / sw.n_tiles;
+  const int64_t tile_id = gid - inst_idx * sw.n_tiles;
+  if (inst_idx >= n_active) return;
+  const InstDev in = insts[inst_idx];
+  sweep_lds_body<R, BLOCK>(sw, ops, terms, in, tile_id, init_tab, n, s, n_sweeps);
 }
 
 // ------------------------------------------------------- register executor
@@ -581,13 +592,16 @@ struct alignas(16) ROp {
   double t, f;   // rotations: ratio t (or u) and scalar factor; form B if regs bit 16
 };
 
+// One tile of one instance through one sweep of register phases (tiles of
+// >= 10 qubits).  `sw` is the sweep (a kernel argument, or a record in
+// device memory for launches that mix plans).
 template <typename R, int MAXT, bool LDSOPS>
-__global__ __launch_bounds__(MAXT) void k_sweep_reg(const DevSweep sw, const DevOp *__restrict__ ops,
-                                                    const DevTerm *__restrict__ terms,
-                                                    const DevPhase *__restrict__ phases,
-                                                    const InstDev *__restrict__ insts, int64_t n_active,
-                                                    const typename CT<R>::V *__restrict__ init_tab, int n, int s,
-                                                    int n_sweeps) {
+__device__ __forceinline__ void sweep_reg_body(const DevSweep &sw, const DevOp *__restrict__ ops,
+                                               const DevTerm *__restrict__ terms,
+                                               const DevPhase *__restrict__ phases, const InstDev &in,
+                                               const int64_t tile_id,
+                                               const typename CT<R>::V *__restrict__ init_tab, int n, int s,
+                                               int n_sweeps) {
   using V = typename CT<R>::V;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V *tile = reinterpret_cast<V *>(smem_raw);
@@ -595,11 +609,6 @@ __global__ __launch_bounds__(MAXT) void k_sweep_reg(const DevSweep sw, const Dev
   ROp *rops = reinterpret_cast<ROp *>(smem_raw + size_t(S) * sizeof(V));
   const int tid = threadIdx.x;
   const int nthr = S >> pqs::kRegBits;
-  const int64_t gid = blockIdx.x;
-  const int64_t inst_idx = gid / sw.n_tiles;
-  const int64_t tile_id = gid - inst_idx * sw.n_tiles;
-  if (inst_idx >= n_active) return;
-  const InstDev in = insts[inst_idx];
 
   uint64_t base = 0;
   for (int r = 0; r < sw.cnruns; ++r)
@@ -912,6 +921,73 @@ __global__ __launch_bounds__(MAXT) void k_sweep_reg(const DevSweep sw, const Dev
       dst[wa] = tile[swz(ja)];
     }
   }
+}
+
+template <typename R, int MAXT, bool LDSOPS>
+__global__ __launch_bounds__(MAXT) void k_sweep_reg(const DevSweep sw, const DevOp *__restrict__ ops,
+                                                    const DevTerm *__restrict__ terms,
+                                                    const DevPhase *__restrict__ phases,
+                                                    const InstDev *__restrict__ insts, int64_t n_active,
+                                                    const typename CT<R>::V *__restrict__ init_tab, int n, int s,
+                                                    int n_sweeps) {
+  const int64_t gid = blockIdx.x;
+  const int64_t inst_idx = gid / sw.n_tiles;
+  const int64_t tile_id = gid - inst_idx * sw.n_tiles;
+  if (inst_idx >= n_active) return;
+  const InstDev in = insts[inst_idx];
+  sweep_reg_body<R, MAXT, LDSOPS>(sw, ops, terms, phases, in, tile_id, init_tab, n, s, n_sweeps);
+}
+
+// ------------------------------------------------- launches that mix plans
+// A device record of a plan for launches that run instances of several
+// plans (uneven widths, depths) at once, and one work item per instance of
+// such a launch: its blocks are the instance's tiles, blocks tile_begin ..
+// tile_begin + n_tiles - 1 of the grid (items sorted by tile_begin).
+struct PlanDev {
+  const DevSweep *sweeps;
+  const DevOp *ops;
+  const DevTerm *terms;
+  const DevPhase *phases;
+  int32_t n_sweeps, n;
+};
+struct MultiItem {
+  int64_t tile_begin;
+  int32_t inst, plan;
+  const void *init_tab;
+};
+
+__device__ __forceinline__ int find_item(const MultiItem *__restrict__ items, int n_items, int64_t b) {
+  int lo = 0, hi = n_items - 1;  // last item with tile_begin <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].tile_begin <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename R, int MAXT, bool LDSOPS>
+__global__ __launch_bounds__(MAXT) void k_sweep_reg_multi(const PlanDev *__restrict__ plans,
+                                                          const InstDev *__restrict__ insts,
+                                                          const MultiItem *__restrict__ items, int n_items, int s) {
+  const int64_t b = blockIdx.x;
+  const MultiItem it = items[find_item(items, n_items, b)];
+  const PlanDev pd = plans[it.plan];
+  const InstDev in = insts[it.inst];
+  sweep_reg_body<R, MAXT, LDSOPS>(pd.sweeps[s], pd.ops, pd.terms, pd.phases, in, b - it.tile_begin,
+                                  static_cast<const typename CT<R>::V *>(it.init_tab), pd.n, s, pd.n_sweeps);
+}
+
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sweep_multi(const PlanDev *__restrict__ plans,
+                                                      const InstDev *__restrict__ insts,
+                                                      const MultiItem *__restrict__ items, int n_items, int s) {
+  const int64_t b = blockIdx.x;
+  const MultiItem it = items[find_item(items, n_items, b)];
+  const PlanDev pd = plans[it.plan];
+  const InstDev in = insts[it.inst];
+  sweep_lds_body<R, BLOCK>(pd.sweeps[s], pd.ops, pd.terms, in, b - it.tile_begin,
+                           static_cast<const typename CT<R>::V *>(it.init_tab), pd.n, s, pd.n_sweeps);
 }
 
 // --------------------------------------------------------------- reductions
@@ -1659,6 +1735,11 @@ int plan_upload(pqs_ctx *ctx, pqs_plan *p) {
   if (!p->dev_phases.empty())
     HIPCHECK(hipMemcpy(p->d_phases, p->dev_phases.data(), p->dev_phases.size() * sizeof(pqs::DevPhase),
                        hipMemcpyHostToDevice));
+  const size_t sb = std::max<size_t>(1, p->dev_sweeps.size()) * sizeof(DevSweep);
+  HIPCHECK(hipMalloc(&p->d_sweeps, sb));
+  if (!p->dev_sweeps.empty())
+    HIPCHECK(hipMemcpy(p->d_sweeps, p->dev_sweeps.data(), p->dev_sweeps.size() * sizeof(DevSweep),
+                       hipMemcpyHostToDevice));
   p->dev_device = ctx->device;
   return PQS_OK;
 }
@@ -1899,6 +1980,7 @@ int pqs_plan_destroy(pqs_plan *p) {
     if (p->d_ops) (void)hipFree(p->d_ops);
     if (p->d_terms) (void)hipFree(p->d_terms);
     if (p->d_phases) (void)hipFree(p->d_phases);
+    if (p->d_sweeps) (void)hipFree(p->d_sweeps);
     for (void *m : p->jit_modules) (void)hipModuleUnload((hipModule_t)m);
   }
   delete p;
@@ -2078,6 +2160,170 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride,
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
   return run_instances(ctx, p, h, init_tab, st);
+}
+
+int pqs_run_multi(pqs_ctx *ctx, pqs_plan *const *plans, int32_t n_plans, const pqs_instance *insts,
+                  const int32_t *inst_plan, const void *const *init_tabs, int64_t n_inst, void *stream) {
+  if (!ctx || !plans || n_plans <= 0) return fail(PQS_ERR_ARG, "NULL ctx or no plans");
+  if (n_inst < 0 || (n_inst > 0 && (!insts || !inst_plan))) return fail(PQS_ERR_ARG, "bad instance arrays");
+  if (n_inst == 0) return PQS_OK;
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int dtype = plans[0] ? plans[0]->dtype : 0;
+  for (int32_t k = 0; k < n_plans; ++k) {
+    pqs_plan *p = plans[k];
+    if (!p) return fail(PQS_ERR_ARG, "NULL plan");
+    if (p->dtype != dtype) return fail(PQS_ERR_ARG, "plans of one multi-plan run share one dtype");
+    int rc = plan_upload(ctx, p);
+    if (rc) return rc;
+  }
+  const size_t ab = amp_bytes(dtype);
+  // launch groups: register-phase plans by tile size (one block size per
+  // launch), the per-op LDS plans (tiles below 10 qubits) together
+  auto group_of = [&](const pqs_plan *p) { return p->reg_exec ? p->tile_bits : 0; };
+  std::vector<InstDev> h(static_cast<size_t>(n_inst));
+  size_t alt_rows_b = 0;
+  std::vector<size_t> alt_off(size_t(n_inst), 0);
+  for (int64_t i = 0; i < n_inst; ++i) {
+    const pqs_instance &a = insts[i];
+    const int32_t pi = inst_plan[i];
+    if (pi < 0 || pi >= n_plans) return fail(PQS_ERR_ARG, "instance plan index out of range");
+    const pqs_plan *p = plans[pi];
+    if (!a.state_dev) return fail(PQS_ERR_ARG, "instance without a state buffer");
+    if (a.init < 0 || a.init > 2) return fail(PQS_ERR_ARG, "instance init must be 0, 1 or 2");
+    if (a.init == 2 && (!init_tabs || !init_tabs[i])) return fail(PQS_ERR_ARG, "product init needs an init table");
+    if (a.tangent >= p->n_tangents) return fail(PQS_ERR_ARG, "tangent index beyond the plan's tangents");
+    if (a.tangent >= 0 && p->n_sub > p->n_tangents)
+      return fail(PQS_ERR_UNSUPPORTED, "tangent runs of plans with split generators: use pqs_run");
+    InstDev d{};
+    d.state = a.state_dev;
+    d.src = a.src_dev;
+    d.params = a.params_dev;
+    d.tangent = a.tangent < 0 ? -1 : a.tangent;
+    d.init = a.init;
+    if (p->rolling) {
+      alt_off[size_t(i)] = alt_rows_b;
+      alt_rows_b += (size_t(1) << p->n) * ab;
+    }
+    h[size_t(i)] = d;
+  }
+  // per (round, group): the active instances, their tiles laid end to end
+  struct Launch {
+    int round, group_k;
+    size_t first, count;
+    int64_t tiles;
+    size_t lds;
+    bool ldsops;
+  };
+  std::vector<MultiItem> items;
+  std::vector<Launch> launches;
+  std::vector<int> groups;
+  int max_s = 0;
+  for (int32_t k = 0; k < n_plans; ++k) {
+    max_s = std::max(max_s, int(plans[k]->dev_sweeps.size()));
+    const int gk = group_of(plans[k]);
+    if (std::find(groups.begin(), groups.end(), gk) == groups.end()) groups.push_back(gk);
+  }
+  for (int r = 0; r < max_s; ++r)
+    for (int gk : groups) {
+      Launch L{r, gk, items.size(), 0, 0, 0, false};
+      for (int64_t i = 0; i < n_inst; ++i) {
+        const pqs_plan *p = plans[inst_plan[i]];
+        if (group_of(p) != gk || r >= int(p->dev_sweeps.size())) continue;
+        const DevSweep &sw = p->dev_sweeps[size_t(r)];
+        MultiItem it{};
+        it.tile_begin = L.tiles;
+        it.inst = int32_t(i);
+        it.plan = inst_plan[i];
+        it.init_tab = init_tabs ? init_tabs[i] : nullptr;
+        items.push_back(it);
+        L.tiles += sw.n_tiles;
+        size_t lds = (size_t(1) << sw.k) * ab;
+        if (p->reg_exec) lds += size_t(sw.op_end - sw.op_begin) * sizeof(ROp);
+        L.lds = std::max(L.lds, lds);
+        L.ldsops = L.ldsops || sw.n_coef;
+      }
+      L.count = items.size() - L.first;
+      if (L.count) launches.push_back(L);
+    }
+  if (items.size() > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "too many instances");
+  std::vector<PlanDev> pdv(static_cast<size_t>(n_plans));
+  for (int32_t k = 0; k < n_plans; ++k) {
+    const pqs_plan *p = plans[k];
+    pdv[size_t(k)] = PlanDev{(const DevSweep *)p->d_sweeps, (const DevOp *)p->d_ops, (const DevTerm *)p->d_terms,
+                             (const DevPhase *)p->d_phases, int32_t(p->dev_sweeps.size()), p->n};
+  }
+  // workspace: [alt rows | instance table | plan records | items]
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t tab_off = al(alt_rows_b);
+  const size_t pd_off = al(tab_off + h.size() * sizeof(InstDev));
+  const size_t it_off = al(pd_off + pdv.size() * sizeof(PlanDev));
+  const size_t total = it_off + items.size() * sizeof(MultiItem);
+  int rc = ws_reserve(ctx, total, st);
+  if (rc) return rc;
+  char *ws = (char *)ctx->ws;
+  for (int64_t i = 0; i < n_inst; ++i) {
+    const pqs_plan *p = plans[inst_plan[i]];
+    if (!p->rolling) continue;
+    InstDev &d = h[size_t(i)];
+    const size_t sb = (size_t(1) << p->n) * ab;
+    d.alt = ws + alt_off[size_t(i)];
+    d.pingpong = 1;
+    const int S = int(p->dev_sweeps.size());
+    if (d.init == 0 && (d.src == nullptr || d.src == d.state)) {
+      if (((S - 1) & 1) == 0) {
+        HIPCHECK(hipMemcpyAsync(d.alt, d.state, sb, hipMemcpyDeviceToDevice, st));
+        d.src = d.alt;
+      } else {
+        d.src = d.state;
+      }
+    }
+  }
+  std::vector<char> blob(total - tab_off, 0);
+  std::memcpy(blob.data(), h.data(), h.size() * sizeof(InstDev));
+  std::memcpy(blob.data() + (pd_off - tab_off), pdv.data(), pdv.size() * sizeof(PlanDev));
+  std::memcpy(blob.data() + (it_off - tab_off), items.data(), items.size() * sizeof(MultiItem));
+  rc = upload(ctx, ws + tab_off, blob.data(), blob.size(), st);
+  if (rc) return rc;
+  const InstDev *d_tab = (const InstDev *)(ws + tab_off);
+  const PlanDev *d_pd = (const PlanDev *)(ws + pd_off);
+  const MultiItem *d_items = (const MultiItem *)(ws + it_off);
+  static bool attr_set[64] = {false};
+  if (ctx->device < 64 && !attr_set[ctx->device]) {
+    HIPCHECK(hipFuncSetAttribute((const void *)k_sweep_reg_multi<double, 512, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+    HIPCHECK(hipFuncSetAttribute((const void *)k_sweep_reg_multi<double, 512, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+    HIPCHECK(hipFuncSetAttribute((const void *)k_sweep_reg_multi<float, 1024, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+    HIPCHECK(hipFuncSetAttribute((const void *)k_sweep_reg_multi<float, 1024, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, ctx->max_lds));
+    attr_set[ctx->device] = true;
+  }
+  for (const Launch &L : launches) {
+    if (L.tiles > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "grid too large: split the batch");
+    if (L.lds > size_t(ctx->max_lds)) return fail(PQS_ERR_UNSUPPORTED, "tile and op records do not fit in LDS");
+    const MultiItem *its = d_items + L.first;
+    const int ni = int(L.count);
+    if (L.group_k > 0) {
+      const unsigned nthr = (1u << L.group_k) >> pqs::kRegBits;
+      if (dtype == PQS_C128) {
+        auto kern = L.ldsops ? k_sweep_reg_multi<double, 512, true> : k_sweep_reg_multi<double, 512, false>;
+        hipLaunchKernelGGL(kern, dim3(unsigned(L.tiles)), dim3(nthr), L.lds, st, d_pd, d_tab, its, ni, L.round);
+      } else {
+        auto kern = L.ldsops ? k_sweep_reg_multi<float, 1024, true> : k_sweep_reg_multi<float, 1024, false>;
+        hipLaunchKernelGGL(kern, dim3(unsigned(L.tiles)), dim3(nthr), L.lds, st, d_pd, d_tab, its, ni, L.round);
+      }
+    } else if (dtype == PQS_C128) {
+      hipLaunchKernelGGL((k_sweep_multi<double, 256>), dim3(unsigned(L.tiles)), dim3(256), L.lds, st, d_pd, d_tab,
+                         its, ni, L.round);
+    } else {
+      hipLaunchKernelGGL((k_sweep_multi<float, 256>), dim3(unsigned(L.tiles)), dim3(256), L.lds, st, d_pd, d_tab,
+                         its, ni, L.round);
+    }
+    HIPCHECK(hipGetLastError());
+  }
+  return PQS_OK;
 }
 
 int64_t pqs_batch_table_bytes(int64_t batch) { return batch < 0 ? 0 : batch * int64_t(sizeof(InstDev)); }

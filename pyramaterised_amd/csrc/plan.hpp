@@ -131,6 +131,7 @@ struct pqs_plan {
   void *d_ops = nullptr;
   void *d_terms = nullptr;
   void *d_phases = nullptr;
+  void *d_sweeps = nullptr;  // DevSweep records (launches that mix plans)
 
   // run-time specialised sweep kernels (hiprtc), per plan on one device
   void *jit_module = nullptr;        // non-null once loaded

@@ -433,6 +433,50 @@ def run(plan, params, init, tangents=None, out=None):
     return out
 
 
+def run_multi(jobs):
+    """Evolve batches of several plans in one multi-plan run (pqs_run_multi):
+    jobs = [(plan, params (B, P) or (P,), Init)] -> list of (B, 2^N_plan)
+    state tensors.  The plans may differ in width and depth (one launch per
+    sweep round and tile size covers all of them)."""
+    ctx = context()
+    dev = device()
+    plans, outs, keep = [], [], []
+    insts, inst_plan, tabs = [], [], []
+    for plan, params, init in jobs:
+        if plan.dtype != jobs[0][0].dtype:
+            raise ValueError("multi-plan runs take plans of one dtype")
+        k = len(plans)
+        plans.append(plan)
+        p = _params_tensor(params, dev, rows=True)
+        if p.numel() == 0:
+            p = torch.zeros((max(1, p.shape[0]), 1), dtype=torch.float64, device=dev)
+        B = p.shape[0]
+        out = torch.empty((B, 1 << plan.n), dtype=CDTYPE[plan.dtype], device=dev)
+        code, src, tab = init.device_args(plan.dtype, dev)
+        keep += [p, src, tab]
+        for b in range(B):
+            inst = L.Instance()
+            inst.state_dev = out[b].data_ptr()
+            inst.src_dev = src.data_ptr() if src is not None else None
+            inst.params_dev = p[b].data_ptr()
+            inst.tangent = -1
+            inst.init = code
+            insts.append(inst)
+            inst_plan.append(k)
+            tabs.append(tab.data_ptr() if tab is not None else None)
+        outs.append(out)
+    n = len(insts)
+    if n == 0:
+        return outs
+    parr = (C.c_void_p * len(plans))(*[pl.handle.value for pl in plans])
+    iarr = (L.Instance * n)(*insts)
+    ipl = (C.c_int32 * n)(*inst_plan)
+    tarr = (C.c_void_p * n)(*tabs)
+    check(L.lib().pqs_run_multi(ctx.handle, parr, len(plans), iarr, ipl, tarr, n, stream_ptr()))
+    _keepalive(*keep)
+    return outs
+
+
 def run_rows(plan, params, states):
     """Evolve every row of a (B, 2^N) state tensor in place through the plan
     (row b with params[b], or one shared (P,) vector) -> states."""

@@ -238,7 +238,7 @@ class Server:
 
     def __init__(self, hbm_budget=None, n_streams=2, max_batch=512, dtype="complex128", keep_states=True,
                  max_inflight_per_stream=2, specialise_qfim_from=14, specialise_state_from=18, compile_workers=8,
-                 large_bytes=1 << 30):
+                 large_bytes=1 << 30, merge_classes=True):
         self.dev = E.device()
         if hbm_budget is None:
             _, total = torch.cuda.mem_get_info(self.dev)
@@ -251,6 +251,7 @@ class Server:
         self.keep_states = keep_states
         self.max_inflight = int(max_inflight_per_stream)
         self.large_bytes = int(large_bytes)
+        self.merge_classes = bool(merge_classes)
         self.slots = [_Slot(self.dev) for _ in range(max(1, n_streams))]
         self._next_slot = 0
         self.circuits = {}
@@ -350,6 +351,43 @@ class Server:
         exceeds large_bytes)."""
         return req.kind == QFIM and self.request_bytes(req) >= self.large_bytes
 
+    def _mergeable(self, key):
+        """A state class that runs on the interpreted kernels (not yet or not
+        to be specialised): it joins the multi-plan launch."""
+        n, depth, kind = key
+        if kind != STATE_MW or not self.merge_classes:
+            return False
+        c = self.circuits.get((n, depth))
+        return c is None or c._jit.get("plan") is None
+
+    def _admit_merged(self, now):
+        """One multi-plan launch (pqs_run_multi) of the oldest admissible
+        state requests of every mergeable class, widths and depths mixed."""
+        slot = self._free_slot(False)
+        if slot is None:
+            return 0
+        take, need = [], 0
+        for key in self._order:
+            if not self._mergeable(key):
+                continue
+            q = self.queues[key]
+            while q and len(take) < self.max_batch:
+                b = self.request_bytes(q[0])
+                if self.in_use + need + b > self.budget:
+                    break
+                take.append(q.popleft())
+                need += b
+            if len(take) >= self.max_batch:
+                break
+        if not take:
+            return 0
+        self._n_pending -= len(take)
+        self._order_dirty = True
+        self.in_use += need
+        self.peak_in_use = max(self.peak_in_use, self.in_use)
+        self._launch_merged(slot, take, need, now)
+        return 1
+
     def _admit_and_launch(self, now):
         launched = 0
         if self._n_pending == 0:
@@ -358,7 +396,11 @@ class Server:
         if self._order_dirty:
             self._order = [k for _, k in sorted((q[0].arrival, k) for k, q in self.queues.items() if q)]
             self._order_dirty = False
+        if self.merge_classes:
+            launched += self._admit_merged(now)
         for key in self._order:
+            if self._mergeable(key):
+                continue
             q = self.queues[key]
             if not q:
                 continue
@@ -422,6 +464,42 @@ class Server:
         self._n_inflight += 1
         self.n_batches += 1
         self.batch_sizes.append(len(reqs))
+
+    def _launch_merged(self, slot, reqs, need, now):
+        """State requests of several classes: one pqs_run_multi over all of
+        them (rows grouped per class), then Meyer-Wallach per class."""
+        groups = OrderedDict()
+        for r in reqs:
+            groups.setdefault((r.n, r.depth), []).append(r)
+        order = [r for g in groups.values() for r in g]
+        for key, rs in groups.items():
+            seen = self._seen[key + (STATE_MW,)] = self._seen.get(key + (STATE_MW,), 0) + len(rs)
+            if seen >= 2 and key[0] >= self.spec_state:
+                self.circuit(*key).specialise("plan", self._pool, self.compile_workers)
+                self._compiling.add(key)
+        b = _Batch()
+        b.reqs, b.bytes, b.slot = order, need, slot
+        b.F = None
+        with E.using(slot.ctx, slot.stream):
+            jobs = []
+            for (n, depth), rs in groups.items():
+                c = self.circuit(n, depth)
+                jobs.append((c.plan, _to_device(np.stack([r.params for r in rs]), self.dev), c.init))
+            outs = E.run_multi(jobs)
+            qs = []
+            b.out = []
+            for ((n, _), rs), out in zip(groups.items(), outs):
+                qs.append(E.mw(out, n)[1])
+                b.out += [out[i] for i in range(len(rs))]
+            b.q = torch.cat(qs)
+            b.event = torch.cuda.Event()
+            b.event.record(slot.stream)
+        for r in order:
+            self.results[r.rid].launched = now
+        slot.inflight.append(b)
+        self._n_inflight += 1
+        self.n_batches += 1
+        self.batch_sizes.append(len(order))
 
     def _retire(self, now):
         done = 0

@@ -134,3 +134,20 @@ def test_specialised_class_matches_alone(sv):
             assert abs(a.Q - b.Q) < 1e-12
         else:
             assert rel_fro(a.qfim, b.qfim) < 1e-12
+
+
+def test_classes_merge_into_multi_plan_launches(sv):
+    """State requests of different widths and depths share launches
+    (pqs_run_multi): far fewer launches than classes, every result equal to
+    the same job run alone."""
+    reqs = sv.make_workload(60, rate=0.0, seed=11, n_range=(6, 16), depth_range=(2, 12), qfim_frac=0.0)
+    classes = {(r.n, r.depth) for r in reqs}
+    srv = sv.Server(n_streams=2, max_batch=32, keep_states=True)
+    mix = srv.run(reqs, realtime=False)
+    srv.close()
+    assert len(classes) > 20
+    assert srv.n_batches <= 4
+    assert max(srv.batch_sizes) >= 16
+    alone, _ = sv.run_one_at_a_time(reqs)
+    for a, b in zip(mix, alone):
+        _check_same(a, b)
