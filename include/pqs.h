@@ -125,11 +125,17 @@ enum {
                                    consecutive qubits, scored by the greedy
                                    pass 1, 2 or 3 sweeps ahead, and keep the
                                    partition with the fewest sweeps) */
-  PQS_PLAN_GREEDY_PHASES = 128  /* take every register phase's qubits from the
+  PQS_PLAN_GREEDY_PHASES = 128, /* take every register phase's qubits from the
                                    greedy pass (default: also try register sets
                                    seeded with 4 consecutive tile qubits and
                                    keep the one after which the greedy pass
                                    needs the fewest phases) */
+  PQS_PLAN_NO_EXCHANGE = 256    /* specialised kernels: every register-phase
+                                   transition goes through LDS (default: a
+                                   phase whose register bits differ from the
+                                   previous phase's by one or two tile bits
+                                   held on lane bits 5 / 4 there trades them
+                                   with cross-lane swaps, no LDS round trip) */
 };
 
 /* One instance of a batched run. */

@@ -139,6 +139,30 @@ template <int B> __device__ __forceinline__ void czc(V (&a)[NA], int on) {
     if (v & (1 << B)) a[v] = V{a[v].x * s, a[v].y * s};
 }
 __device__ __forceinline__ void negall(V (&a)[NA], int on) { rscale(a, on ? R(-1) : R(1)); }
+// Trade lane bit LB (5 or 4) with a register bit: x holds the register bit's
+// 0 half, y its 1 half; y of a lane with the lane bit clear and x of its
+// partner (the bit set) swap places (v_permlane32_swap / v_permlane16_swap
+// on every 32-bit word).
+template <int LB> __device__ __forceinline__ void lswap(V &x, V &y) {
+  constexpr int nw = int(sizeof(V) / 4);
+  unsigned xs[nw], ys[nw];
+  __builtin_memcpy(xs, &x, sizeof(V));
+  __builtin_memcpy(ys, &y, sizeof(V));
+#pragma unroll
+  for (int i = 0; i < nw; ++i) {
+    if constexpr (LB == 5) {
+      const auto r = __builtin_amdgcn_permlane32_swap(xs[i], ys[i], false, false);
+      xs[i] = r[0];
+      ys[i] = r[1];
+    } else {
+      const auto r = __builtin_amdgcn_permlane16_swap(xs[i], ys[i], false, false);
+      xs[i] = r[0];
+      ys[i] = r[1];
+    }
+  }
+  __builtin_memcpy(&x, xs, sizeof(V));
+  __builtin_memcpy(&y, ys, sizeof(V));
+}
 )PQS";
 
 static constexpr int NA_() { return 1 << kRegBits; }
@@ -341,6 +365,9 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     // is applied at the end of the sweep, at the end of a phase with
     // per-lane sign flips, or early by the growth check
     o << "  double fs = 1.0;\n";
+    // registers carried into a phase that takes its layout by an exchange
+    o << "  V areg[NA];\n";
+    bool carry = false;
     int nrot = 0;  // rotations since the last check of the accumulated scale
     // phases
     for (int ph = ds.phase_begin; ph < ds.phase_end; ++ph) {
@@ -359,7 +386,10 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         kv[v] = a;
       }
       o << "    const int ptb = pt * int(sizeof(V));\n    V a[NA];\n";
-      if (dload && ph == ds.phase_begin) {
+      if (carry) {
+        for (int v = 0; v < NA_(); ++v) o << "    a[" << v << "] = areg[" << v << "];\n";
+        carry = false;
+      } else if (dload && ph == ds.phase_begin) {
         for (int v = 0; v < NA_(); ++v) o << "    a[" << v << "] = pre[" << v << "];\n";
       } else {
         for (int v = 0; v < NA_(); ++v)
@@ -493,6 +523,27 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
           default:
             break;
         }
+      }
+      // Exchange into the next phase: its layout is this one with register
+      // bits traded against lane bits 5 / 4, so the amplitudes move with
+      // cross-lane swaps and stay in registers (no LDS round trip, no
+      // barrier).  Pending flips are applied as data first.
+      const bool xnext = ph + 1 < ds.phase_end && p.dev_phases[ph + 1].xchg > 0 && !p.dev_phases[ph + 1].lds;
+      if (xnext) {
+        for (int r = 0; r < kRegBits; ++r) materialise(r);
+        if (lane_sign) {
+          o << "    rscale(a, R(fs));\n    fs = 1.0;\n";
+          nrot = 0;
+        }
+        const DevPhase &Q = p.dev_phases[ph + 1];
+        for (int j = 0; j < Q.xchg; ++j)
+          for (int v = 0; v < NA_(); ++v)
+            if (!(v & (1 << Q.xr[j])))
+              o << "    lswap<" << Q.xb[j] << ">(a[" << v << "], a[" << (v | (1 << Q.xr[j])) << "]);\n";
+        for (int v = 0; v < NA_(); ++v) o << "    areg[" << v << "] = a[" << v << "];\n";
+        o << "  }\n";
+        carry = true;
+        continue;
       }
       // stores land where the flipped indices belong
       {

@@ -731,8 +731,36 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         i = j;
       }
     }
+    // Register tile bits of phase j after a register phase with register
+    // tile bits prev (0: none): the phase's qubits, then filler bits, taken
+    // first from prev (amplitudes that stay in their registers across the
+    // transition), then the lowest other tile bits; never wave bits.
+    auto regset_of = [&](size_t j, int prev) {
+      int m = 0, nr = 0;
+      for (int q = 0; q < n; ++q)
+        if (((phases[j].regs >> q) & 1) && tile_index[q] >= 0) {
+          m |= 1 << tile_index[q];
+          ++nr;
+        }
+      for (int pass = 0; pass < 2; ++pass)
+        for (int t = 0; t < ds.k && nr < kRegBits; ++t)
+          if (!((m >> t) & 1) && !((phase_w[j] >> t) & 1) && (pass == 1 || ((prev >> t) & 1))) {
+            m |= 1 << t;
+            ++nr;
+          }
+      return m;
+    };
+    // Exchange transitions (PQS_PLAN_NO_EXCHANGE off): a register phase whose
+    // register set differs from the previous one's by at most two tile bits,
+    // all thread bits of the previous phase below the wave bits, with the
+    // same wave bits, takes the previous layout with those bits traded
+    // against lane bits 5 and 4; the previous phase puts them there.
+    const bool exch_ok = !(p.flags & PQS_PLAN_NO_EXCHANGE);
+    std::vector<int> xnew(phases.size(), 0);  // per phase: planned exchange bits (tile-bit mask), 0 none
+    int prev_regs = 0;                       // register tile bits of the previous register phase
     size_t phase_no = 0;
     for (const PhaseSpec &ph : phases) {
+      const size_t idx = phase_no;
       const bool first = phase_no == 0, last = phase_no + 1 == phases.size();
       const int wmask = phase_w[phase_no++];
       DevPhase dp{};
@@ -740,9 +768,46 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
       dp.lds = ph.lds ? 1 : 0;
       int reg_index[kMaxTileBits];
       for (int t = 0; t < kMaxTileBits; ++t) reg_index[t] = -1;
-      if (p.reg_exec && !ph.lds) {
-        // register bits: the phase's qubits, then the lowest other tile bits
-        // (not wave bits)
+      if (p.reg_exec && !ph.lds && xnew[idx]) {
+        // the previous layout with the planned bits traded (register bit of
+        // a displaced tile bit <-> the lane bit holding a new one)
+        const DevPhase &pv = p.dev_phases.back();
+        const int rq = regset_of(idx, prev_regs);
+        dp.ntb = pv.ntb;
+        for (int i = 0; i < kRegBits; ++i) dp.rb[i] = pv.rb[i];
+        for (int i = 0; i < pv.ntb; ++i) dp.tb[i] = pv.tb[i];
+        // the lanes that hold the new register bits (5, 4, or both)
+        int lanes[2], nl = 0;
+        for (int l = 5; l >= 4; --l)
+          if ((rq >> pv.tb[l]) & 1) lanes[nl++] = l;
+        int nd = 0;
+        for (int i = 0; i < kRegBits; ++i) {
+          if ((rq >> pv.rb[i]) & 1) continue;  // stays a register bit
+          if (nd >= nl) return fail(PQS_ERR_UNSUPPORTED, "internal planner error: exchange lanes");
+          const int lane = lanes[nd];
+          dp.rb[i] = pv.tb[lane];
+          dp.tb[lane] = pv.rb[i];
+          dp.xr[nd] = i;
+          dp.xb[nd] = lane;
+          ++nd;
+        }
+        dp.xchg = nd;
+        int got = 0;
+        for (int i = 0; i < kRegBits; ++i) got |= 1 << dp.rb[i];
+        if (got != rq || nd < 1 || nd > 2)
+          return fail(PQS_ERR_UNSUPPORTED, "internal planner error: exchange transition mismatch");
+        for (int i = 0; i < kRegBits; ++i) reg_index[dp.rb[i]] = i;
+        if (direct_ok && last && wbit[0] >= 0 && wbit[1] >= 0) {
+          bool l0 = false, l1 = false;
+          for (int i = 0; i < std::min(6, dp.ntb); ++i) {
+            l0 = l0 || dp.tb[i] == wbit[0];
+            l1 = l1 || dp.tb[i] == wbit[1];
+          }
+          if (l0 && l1) ds.direct_store = 1;
+        }
+      } else if (p.reg_exec && !ph.lds) {
+        // register bits: the phase's qubits, then filler bits (regset_of)
+        const int rmask = regset_of(idx, prev_regs);
         int nr = 0;
         for (int q = 0; q < n; ++q)
           if ((ph.regs >> q) & 1) {
@@ -752,10 +817,11 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
             dp.rb[nr++] = tile_index[q];
           }
         for (int t = 0; t < ds.k && nr < kRegBits; ++t)
-          if (reg_index[t] < 0 && !((wmask >> t) & 1)) {
+          if (reg_index[t] < 0 && ((rmask >> t) & 1)) {
             reg_index[t] = nr;
             dp.rb[nr++] = t;
           }
+        if (nr != kRegBits) return fail(PQS_ERR_UNSUPPORTED, "internal planner error: register bits short");
         // Thread bits; the first ones become lane bits 0, 1, 2, ... and are
         // ordered so that their images under the LDS swizzle (tile bit t
         // lands on bank-group bit t mod 4) differ, spreading each group of
@@ -828,6 +894,64 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         for (int t : rest) dp.tb[dp.ntb++] = t;
         for (int t = 0; t < ds.k; ++t)
           if ((wmask >> t) & 1) dp.tb[dp.ntb++] = t;
+        // Plan an exchange into the next phase: its new register bits go to
+        // lane bits 5 and 4 here (each may take either).  A bit already on
+        // lane 3-5 moves freely; from lanes 0-2 a move is preferred against
+        // a bit of its bank class (the lane order's conflict rule), else it
+        // costs the phase's LDS accesses a possible conflict, which an
+        // exchange more than repays (it saves a whole LDS round trip).  The
+        // lanes a direct load of this phase needs stay put.
+        if (exch_ok && idx + 1 < phases.size() && !phases[idx + 1].lds && phase_w[idx + 1] == wmask &&
+            dp.ntb >= 6) {
+          int rp = 0;
+          for (int i = 0; i < kRegBits; ++i) rp |= 1 << dp.rb[i];
+          const int rq = regset_of(idx + 1, rp);
+          const int nb = rq & ~rp;
+          int nbits[2], cnt = 0;
+          for (int t = 0; t < ds.k; ++t)
+            if ((nb >> t) & 1) {
+              if (cnt < 2) nbits[cnt] = t;
+              ++cnt;
+            }
+          int protect = 0;
+          if (first && direct_ok)
+            for (int i = 0; i < 4; ++i)
+              if (rbit[i] >= 0) protect |= 1 << rbit[i];
+          int best_cost = 1 << 30;
+          DevPhase best = dp;
+          const int lanes_of[2][2] = {{5, 4}, {4, 5}};
+          for (int order = 0; cnt >= 1 && cnt <= 2 && order < 2; ++order) {
+            DevPhase trial = dp;
+            int cost = 0;
+            bool ok = true;
+            for (int j = 0; ok && j < cnt; ++j) {
+              const int t = nbits[j], lane = lanes_of[order][j];
+              int pos = -1;
+              for (int i = 0; i < 6; ++i)
+                if (trial.tb[i] == t) pos = i;
+              if (pos < 0 || ((protect >> t) & 1)) {
+                ok = false;
+                break;
+              }
+              if (pos == lane) continue;
+              const int other = trial.tb[lane];
+              if ((protect >> other) & 1) {
+                ok = false;
+                break;
+              }
+              if (pos < 3 && (other & 3) != (t & 3)) ++cost;
+              std::swap(trial.tb[pos], trial.tb[lane]);
+            }
+            if (ok && cost < best_cost) {
+              best_cost = cost;
+              best = trial;
+            }
+          }
+          if (best_cost < (1 << 30)) {
+            dp = best;
+            xnew[idx + 1] = nb;
+          }
+        }
         if (direct_ok) {
           auto is_lane = [&](int t) {
             for (int i = 0; i < std::min(6, dp.ntb); ++i)
@@ -839,6 +963,12 @@ int build_plan(pqs_plan &p, int n, const pqs_op *ops, int n_ops, const pqs_plan_
         }
         if (wmask && dp.ntb != ds.k - kRegBits)
           return fail(PQS_ERR_UNSUPPORTED, "internal planner error: wave bits overlap the register bits");
+      }
+      if (p.reg_exec && !ph.lds) {
+        prev_regs = 0;
+        for (int i2 = 0; i2 < kRegBits; ++i2) prev_regs |= 1 << dp.rb[i2];
+      } else {
+        prev_regs = 0;
       }
       if (p.reg_exec && !ph.lds) {
         // internal consistency: register + thread bits permute the tile bits

@@ -70,6 +70,13 @@ struct DevPhase {
   int32_t rb[kRegBits];      // tile-local bit of register bit i
   int32_t ntb;               // number of thread bits
   int32_t tb[kMaxTileBits];  // tile-local bit of thread bit i
+  // Exchange with the previous register phase: this phase's layout is the
+  // previous one's with register bit xr[j] and thread (lane) bit xb[j]
+  // (4 or 5) traded, j < xchg, and nothing else changed, so a specialised
+  // kernel moves the amplitudes with cross-lane swaps instead of an LDS
+  // round trip.  0: an ordinary transition.
+  int32_t xchg;
+  int32_t xr[2], xb[2];
 };
 
 struct DevSweep {

@@ -147,13 +147,16 @@ def tile_candidates(n, dtype_c):
     block's LDS phases overlap the other's HBM traffic) and 11-bit tiles
     (five blocks per CU) with rolling layouts (the tile moves freely over
     the qubits; out of place) and 128 B or 256 B write runs, or fixed low
-    qubits (in place).  c64: 11- to 13-bit tiles (16-64 KiB).  Which wins
+    qubits (in place); 12-bit rolling tiles with and without exchange
+    transitions.  c64: 11- to 13-bit tiles (16-64 KiB).  Which wins
     depends on the circuit and the batch: deep sweeps favour more resident
     blocks, shallow HBM-bound ones larger tiles.
     """
     R = L.PQS_PLAN_ROLLING
     if dtype_c == L.PQS_C128:
-        c = [(11, 3, R), (12, 4, R), (12, 3, R), (12, 4, 0)]
+        # register-phase transitions by cross-lane swaps save LDS round trips
+        # but keep flips and scales materialised across them: timed both ways
+        c = [(11, 3, R), (12, 4, R), (12, 4, R | L.PQS_PLAN_NO_EXCHANGE), (12, 3, R), (12, 4, 0)]
     else:
         c = [(12, 4, R), (13, 4, R), (11, 4, R), (13, 4, 0)]
     out = []

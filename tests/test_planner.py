@@ -737,3 +737,26 @@ def test_adjoint_program_refuses_non_unitary_gates(pyqc):
     qc.add_layer([pyqc.R_x(0, 2), pyqc.sqrtH(0, 2), pyqc.R_y(1, 2)])
     with pytest.raises(NotImplementedError):
         adjoint.AdjointProgram(qc)
+
+
+def test_exchange_transitions_compile_and_keep_layouts(pyqc, tmp_path, monkeypatch):
+    """The headline circuit's 11-bit rolling plan trades register and lane
+    bits between a dozen register phases (cross-lane swaps in the
+    specialised kernels); its sources compile with hiprtc, and with
+    PQS_PLAN_NO_EXCHANGE no phase exchanges and no swap is emitted."""
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    monkeypatch.setenv("PQS_JIT_CACHE", str(tmp_path))
+    n = 16
+    qc = pyqc.PQC(n, jit=False)
+    for l in T.clifford_HE_layers(20, n):
+        qc.add_layer(l)
+    R = L.PQS_PLAN_ROLLING
+    plan = E.Plan(n, qc.ops(), tile_bits=11, low_bits=3, flags=R)
+    srcs = plan.jit_sweep_sources()
+    n_swaps = sum(s.count(b"lswap<") for s in srcs)  # calls (the prelude defines lswap( )
+    assert n_swaps >= 8 * 8  # at least 8 exchanged register bits (8 pairs each)
+    plan2 = E.Plan(n, qc.ops(), tile_bits=11, low_bits=3, flags=R | L.PQS_PLAN_NO_EXCHANGE)
+    assert sum(s.count(b"lswap<") for s in plan2.jit_sweep_sources()) == 0
+    E._compile_in_children(srcs[:2], 2)
+    plan.jit(load=False, workers=1)
