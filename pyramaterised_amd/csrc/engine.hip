@@ -1231,121 +1231,199 @@ __global__ __launch_bounds__(256) void k_gram_partial(const typename CT<R>::V *_
 }
 
 // Real symmetric rank-K update C = R R^T (R: P x K doubles, row stride ld),
-// upper 128 x 128 tiles only, split over K, on the FP64 matrix cores.
-constexpr int SYT = 128;
-
-// v_mfma_f64_16x16x4f64 takes,
-// in lane l, A[l % 16][l / 16] and B[l / 16][l % 16] and returns
-// D[4 r + l / 16][l % 16] in element r.  256 threads: wave w owns the 64 x 64
-// quadrant (w / 2, w % 2) of the 128 x 128 tile as 4 x 4 blocks of 16 x 16.
-// K advances in steps of 16: the rows' k-slices are staged row-major in LDS
-// (row stride 18 doubles: the 32 lanes of one operand read hit 64 distinct
-// banks) and the next step's slices are loaded into registers while the
-// current step runs.  Accumulators 128 registers, occupancy 2 waves / SIMD.
+// upper 64 x 64 tiles only, split over K, on the FP64 matrix cores.
+//
+// v_mfma_f64_16x16x4f64 takes, in lane l, A[l % 16][l / 16] and
+// B[l / 16][l % 16] and returns D[4 r + l / 16][l % 16] in element r.  The
+// sum over k does not care which k sits in which of the four k slots, so
+// lane l (row l % 16 of a 16-row block, k group g = l / 16) loads the pair
+// R[row][kk + 2 g + {0, 1}] straight from memory into registers and feeds
+// element s of the pair to the product of k slot g in sub-step s; B uses the
+// same map, so the products pair up.  No LDS and no barriers: every wave is
+// an independent unit of work (one 64 x 64 tile of C over one K chunk, 4 x 4
+// blocks of 16 x 16, 128 accumulator registers), and each 8-wide k step's
+// loads are issued a step ahead of its products.  Diagonal tiles load one
+// operand, skip the 16 x 16 blocks below the diagonal and with the spare
+// matrix-core time form the QFIM's b = <J_p|psi> for their 64 rows (psi as
+// a second operand with two non-zero columns).
 typedef double d4_t __attribute__((ext_vector_type(4)));
-constexpr int MFK = 16, MFLD = MFK + 2;
+// Two waves per SIMD (249 registers each) and a two-step load ring: one
+// wave per SIMD with a ring of 4 or 6 steps in the accumulator registers ran
+// 5-10% slower, a ring of 3 at two waves spills.
+constexpr int SYW = 64, SYK = 8, SYWAVES = 2, SYRING = 2;
+// Jacobians above kQfimStageBytes go through the product in column blocks of
+// kQfimBlockBytes copied to a compact buffer (below it the copy costs more
+// than the shorter row stride gains).
+constexpr size_t kQfimStageBytes = size_t(1) << 30, kQfimBlockBytes = size_t(512) << 20;
 
-// Work of one block: the XCD-aware map from the linear block index.  The
-// dispatcher deals consecutive workgroups round-robin to the 8 XCDs, each
-// with its own L2; chunk c of the K split belongs to XCD c % 8, and every
-// tile of a chunk runs on that XCD, so the chunk's rows are fetched from HBM
-// once and re-read (by the other tiles sharing them) from that XCD's L2.
-// Blocks past an XCD's share (ks not a multiple of 8) return at once.
-struct SyrkWork {
-  int tile, chunk;
-};
-__device__ __forceinline__ SyrkWork syrk_work(int64_t L, int64_t T, int64_t ks) {
-  const int64_t x = L & 7, j = L >> 3;
-  const int64_t n_x = (ks - x + 7) >> 3;  // chunks x, x + 8, ... below ks
-  if (j >= n_x * T) return SyrkWork{-1, -1};
-  return SyrkWork{int(j % T), int(x + 8 * (j / T))};
+template <typename TIn>
+__device__ __forceinline__ double2 syrk_ld2(const char *p) {
+  if constexpr (sizeof(TIn) == 8) {
+    return *reinterpret_cast<const double2 *>(p);
+  } else {
+    const float2 f = *reinterpret_cast<const float2 *>(p);
+    return double2{double(f.x), double(f.y)};
+  }
 }
 
-// TIn = double (complex128 Jacobian) or float (complex64: operands widened
-// to double in registers, so products and sums are the FP64 path's).
+// Operand addresses are a uniform base (the k step) plus a 32-bit byte
+// offset per lane (its row and k group; the host keeps R under 4 GiB per
+// launch).  Every chunk is a whole number of ring turns (the host pads K),
+// so no step needs masking; the last turn's prefetches (never used) re-read
+// the chunk's first step instead of running past it.
+template <typename TIn, bool DIAG>
+__device__ __forceinline__ void syrk_wave_body(const TIn *R, const uint32_t (&oa)[4], const uint32_t (&ob)[4],
+                                               const TIn *psi, int64_t k0, int64_t k1, int g, int col,
+                                               d4_t (&acc)[4][4], d4_t (&accb)[4]) {
+  double2 A[SYRING][4], B[SYRING][4];
+  auto load = [&](double2(&X)[4], double2(&Y)[4], int64_t kk) {
+    const int64_t kl = kk < k1 ? kk : k0;
+    const char *base = reinterpret_cast<const char *>(R + kl);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) X[a] = syrk_ld2<TIn>(base + oa[a]);
+    if constexpr (!DIAG) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) Y[b] = syrk_ld2<TIn>(base + ob[b]);
+    } else {
+      // the b vector's operand: column 0 pairs (Re, Im) of a row with
+      // (Re psi, Im psi), column 1 with (Im psi, -Re psi), the rest zero
+      const double2 v = syrk_ld2<TIn>(reinterpret_cast<const char *>(psi + kl + 2 * g));
+      Y[0] = col == 0 ? v : col == 1 ? double2{v.y, -v.x} : double2{0.0, 0.0};
+    }
+  };
+  // the two sub-steps in turn over all blocks: products into one
+  // accumulator are 16 (14) instructions apart
+  auto mma = [&](const double2(&X)[4], const double2(&Y)[4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (DIAG && a > b) continue;
+        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[a].x, DIAG ? X[b].x : Y[b].x, acc[a][b], 0, 0, 0);
+      }
+    if constexpr (DIAG) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) accb[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[a].x, Y[0].x, accb[a], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (DIAG && a > b) continue;
+        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[a].y, DIAG ? X[b].y : Y[b].y, acc[a][b], 0, 0, 0);
+      }
+    if constexpr (DIAG) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) accb[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[a].y, Y[0].y, accb[a], 0, 0, 0);
+    }
+  };
+#pragma unroll
+  for (int r = 0; r < SYRING; ++r) load(A[r], B[r], k0 + r * SYK);
+  for (int64_t kk = k0; kk < k1; kk += SYRING * SYK) {
+#pragma unroll
+    for (int r = 0; r < SYRING; ++r) {
+      mma(A[r], B[r]);
+      load(A[r], B[r], kk + (r + SYRING) * SYK);
+    }
+  }
+}
+
+// Work of one wave: the XCD-aware map from (block, wave).  The dispatcher
+// deals consecutive workgroups round-robin to the 8 XCDs, each with its own
+// L2.  Items (tile, K chunk) are numbered chunk-major and XCD x runs the
+// contiguous range [x per_xcd, (x + 1) per_xcd): the tiles of a chunk run
+// side by side on one XCD and walk its k range together, so a chunk's rows
+// are fetched from HBM about once and re-read from that XCD's L2.
 template <typename TIn>
-__global__ __launch_bounds__(256, 2) void k_syrk_mfma(const TIn *__restrict__ R, int64_t ld, int64_t P, int64_t K,
-                                                      int64_t kchunk, int64_t ks, const int2 *__restrict__ tiles,
-                                                      int64_t T, double *__restrict__ part) {
-  __shared__ double sa[SYT * MFLD];
-  __shared__ double sb[SYT * MFLD];
-  const SyrkWork wk = syrk_work(blockIdx.x, T, ks);
-  if (wk.tile < 0) return;
-  const int2 tile = tiles[wk.tile];
-  const int64_t i0 = int64_t(tile.x) * SYT, j0 = int64_t(tile.y) * SYT;
-  const int64_t k0 = int64_t(wk.chunk) * kchunk;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SYWAVES))) void k_syrk_mfma(
+    const TIn *__restrict__ R, int64_t ld, int64_t P, int64_t K, int64_t kchunk, int64_t n_items, int64_t per_xcd,
+    const int2 *__restrict__ tiles, int64_t T, const TIn *__restrict__ psi, int accum, double *__restrict__ part,
+    double2 *__restrict__ bpart) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, r16 = l & 15;
+  const int64_t slot = int64_t(blockIdx.x >> 3) * 4 + w;
+  if (slot >= per_xcd) return;
+  const int64_t m = int64_t(blockIdx.x & 7) * per_xcd + slot;
+  if (m >= n_items) return;
+  const int64_t chunk = m / T;
+  const int2 tile = tiles[m - chunk * T];
+  const int64_t i0 = int64_t(tile.x) * SYW, j0 = int64_t(tile.y) * SYW;
+  const int64_t k0 = chunk * kchunk;
+  if (k0 >= K) return;  // a narrower last column block has fewer chunks
   const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
-  const int t = threadIdx.x, l = t & 63, w = t >> 6;
-  const int qr = (w >> 1) * 64, qc = (w & 1) * 64;
+  // rows past P read row P - 1: they only feed entries that are not stored
+  uint32_t oa[4], ob[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t ra = i0 + 16 * a + r16, rb = j0 + 16 * a + r16;
+    oa[a] = uint32_t(((ra < P ? ra : P - 1) * ld + 2 * g) * int64_t(sizeof(TIn)));
+    ob[a] = uint32_t(((rb < P ? rb : P - 1) * ld + 2 * g) * int64_t(sizeof(TIn)));
+  }
   d4_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
-  // staging: element e = t + 256 u is row e / 8, values 2 (e % 8) + {0, 1}
-  // of the step's 16 (K, kchunk and the steps are even, so a pair is
-  // either inside [k0, k1) or outside it)
-  double2 pa[4], pb[4];
-  auto ld2 = [&](int64_t row, int64_t k) -> double2 {
-    if constexpr (sizeof(TIn) == 8) {
-      return *reinterpret_cast<const double2 *>(R + row * ld + k);
-    } else {
-      const float2 f = *reinterpret_cast<const float2 *>(R + row * ld + k);
-      return double2{double(f.x), double(f.y)};
-    }
-  };
-  auto gload = [&](int64_t kk) {
+  d4_t accb[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = t + 256 * u, row = e >> 3, c2 = e & 7;
-      const int64_t k = kk + 2 * c2, ra = i0 + row, rb = j0 + row;
-      pa[u] = (ra < P && k < k1) ? ld2(ra, k) : double2{0.0, 0.0};
-      pb[u] = (rb < P && k < k1) ? ld2(rb, k) : double2{0.0, 0.0};
-    }
-  };
-  if (k0 < k1) gload(k0);
-  for (int64_t kk = k0; kk < k1; kk += MFK) {
-    __syncthreads();  // the previous step's operand reads are done
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = t + 256 * u, row = e >> 3, c2 = e & 7;
-      *reinterpret_cast<double2 *>(&sa[row * MFLD + 2 * c2]) = pa[u];
-      *reinterpret_cast<double2 *>(&sb[row * MFLD + 2 * c2]) = pb[u];
-    }
-    __syncthreads();
-    if (kk + MFK < k1) gload(kk + MFK);
-#pragma unroll
-    for (int k4 = 0; k4 < MFK; k4 += 4) {
-      double af[4], bf[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) af[a] = sa[(qr + 16 * a + (l & 15)) * MFLD + k4 + (l >> 4)];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) bf[b] = sb[(qc + 16 * b + (l & 15)) * MFLD + k4 + (l >> 4)];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
-    }
-  }
-  double *out = part + int64_t(wk.chunk) * P * P;
+  for (int a = 0; a < 4; ++a) accb[a] = d4_t{0.0, 0.0, 0.0, 0.0};
+  const bool diag = tile.x == tile.y;
+  if (diag)
+    syrk_wave_body<TIn, true>(R, oa, ob, psi, k0, k1, g, r16, acc, accb);
+  else
+    syrk_wave_body<TIn, false>(R, oa, ob, psi, k0, k1, g, r16, acc, accb);
+  double *out = part + chunk * P * P;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < 4; ++b) {
+      if (diag && a > b) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + qr + 16 * a + 4 * r + (l >> 4), j = j0 + qc + 16 * b + (l & 15);
-        if (i < P && j < P) out[i * P + j] = acc[a][b][r];
+        const int64_t i = i0 + 16 * a + 4 * r + g, j = j0 + 16 * b + r16;
+        if (i < P && j < P) out[i * P + j] = accum ? out[i * P + j] + acc[a][b][r] : acc[a][b][r];
       }
+    }
+  if (diag && r16 < 2) {
+    // <row|psi> over the chunk: column 0 (lanes 0, 16, 32, 48) the real
+    // part, column 1 the imaginary part
+    double *bo = reinterpret_cast<double *>(bpart + chunk * P);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + 16 * a + 4 * r + g;
+        if (i < P) bo[2 * i + r16] = accum ? bo[2 * i + r16] + accb[a][r] : accb[a][r];
+      }
+  }
 }
 
-// Column-major C[i + j P] = sum over the split of part[z][i][j], for entries
-// of computed (upper) tiles.
+// dst[r][0, w) = src[r][0, w) for P rows (16-byte units, row strides lds and
+// ldd): a column block of the Jacobian into a compact buffer.
+__global__ void k_copy_cols(const uint4 *__restrict__ src, int64_t lds, uint4 *__restrict__ dst, int64_t ldd,
+                            int64_t w, int64_t units) {
+  const int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= units) return;
+  const int64_t r = u / w, c = u - r * w;
+  dst[r * ldd + c] = src[r * lds + c];
+}
+
+// dst[r][c] = src[r][c] for c < ld_src, zero up to ld_dst (units = P ld_dst
+// elements): a short Jacobian padded to whole k steps.
+template <typename T>
+__global__ void k_pad_cols(const T *__restrict__ src, int64_t lds, T *__restrict__ dst, int64_t ldd, int64_t units) {
+  const int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= units) return;
+  const int64_t r = u / ldd, c = u - r * ldd;
+  dst[u] = c < lds ? src[r * lds + c] : T(0);
+}
+
+// Column-major C[i + j P] = sum over the split of part[z][i][j], for the
+// upper triangle i <= j (every such entry lies in a computed block).
 __global__ void k_syrk_reduce(const double *__restrict__ part, int64_t P, int64_t ks, double *__restrict__ C) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= P * P) return;
   const int64_t i = e / P, j = e - i * P;
-  if (i / SYT > j / SYT) return;
+  if (i > j) return;
   double a = 0;
   for (int64_t z = 0; z < ks; ++z) a += part[z * P * P + e];
   C[i + j * P] = a;
@@ -2677,44 +2755,106 @@ int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void
     // in P, and computes both triangles).  complex64 Jacobians are widened
     // to double in registers: the same products and sums.
     const int64_t K = 2 * dim;
-    const int64_t nt = (P + SYT - 1) / SYT;
+    const size_t esz = dtype == PQS_C128 ? sizeof(double) : sizeof(float);
+    const int64_t nt = (P + SYW - 1) / SYW;
     std::vector<int2> tl;
     for (int64_t a = 0; a < nt; ++a)
       for (int64_t b = a; b < nt; ++b) tl.push_back(int2{int(a), int(b)});
     const int64_t T = int64_t(tl.size());
-    // enough blocks for two per CU, the split a multiple of the 8 XCDs
-    int64_t ks = (2 * int64_t(ctx->n_cu) + T - 1) / T;
-    ks = (ks + 7) / 8 * 8;
-    ks = std::max<int64_t>(1, std::min<int64_t>(ks, K / 256));
-    int64_t kchunk = (K + ks - 1) / ks;
-    kchunk = (kchunk + MFK - 1) / MFK * MFK;
-    ks = (K + kchunk - 1) / kchunk;
-    const int64_t grid = 8 * ((ks + 7) / 8) * T;
+    // A Jacobian above 1 GiB is taken in column blocks of at most 512 MiB,
+    // each copied to a compact P x Kc buffer first: rows of 2^20 amplitudes
+    // and more lie 16 MiB apart, and every load of the product kernel
+    // touches 16 rows, so the TLB misses on nearly every access (measured:
+    // 23% of translations missed, 12x the Jacobian fetched from HBM).  The
+    // copy costs one extra read and write of J.
+    const int64_t ring = SYRING * SYK;
+    int64_t Kc = K;
+    const bool pad = K % ring != 0;  // fewer than 8 amplitudes: zero-padded copy
+    const bool staged = pad || size_t(P) * size_t(K) * esz > kQfimStageBytes;
+    if (pad) {
+      Kc = (K + ring - 1) / ring * ring;
+    } else if (staged) {
+      Kc = int64_t(kQfimBlockBytes / (size_t(P) * esz)) / ring * ring;
+      Kc = std::max<int64_t>(ring, std::min<int64_t>(Kc, K));
+    }
+    // one item per wave slot (a single round), chunks of at least 1024 k, a
+    // multiple of the load ring
+    const int64_t slots = 4 * SYWAVES * int64_t(ctx->n_cu);
+    int64_t ks = std::max<int64_t>(1, std::min<int64_t>(slots / T, Kc / 1024));
+    int64_t kchunk = (Kc + ks - 1) / ks;
+    kchunk = (kchunk + ring - 1) / ring * ring;
+    ks = (Kc + kchunk - 1) / kchunk;
+    const int64_t n_items = T * ks, per_xcd = (n_items + 7) / 8;
+    const int64_t grid = 8 * ((per_xcd + 3) / 4);
     if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "QFIM too large");
-    const size_t part_b = std::max(gram_part_bytes(ctx, 1, P, dim), size_t(ks * P * P) * sizeof(double));
-    const size_t off_c = (part_b + 255) & ~size_t(255);
-    const size_t off_b = (off_c + size_t(P * P) * sizeof(double) + 255) & ~size_t(255);
-    const size_t off_t = (off_b + size_t(P) * sizeof(double2) + 255) & ~size_t(255);
-    int rc = ws_reserve(ctx, off_t + tl.size() * sizeof(int2), st);
+    // workspace: [partials ks P^2 | C P^2 | b P | b partials ks P | tiles |
+    // staged block (P Kc) | padded psi (Kc, short Jacobians only)]
+    const size_t al = 255;
+    const size_t off_c = (size_t(ks * P * P) * sizeof(double) + al) & ~al;
+    const size_t off_b = (off_c + size_t(P * P) * sizeof(double) + al) & ~al;
+    const size_t off_bp = (off_b + size_t(P) * sizeof(double2) + al) & ~al;
+    const size_t off_t = (off_bp + size_t(ks * P) * sizeof(double2) + al) & ~al;
+    const size_t off_s = (off_t + tl.size() * sizeof(int2) + al) & ~al;
+    // staged rows are Kc + 32 elements apart: a power-of-two row stride
+    // would put the 16 rows of every load on one L2 channel
+    const int64_t Ks = Kc + 32;
+    const size_t off_ps = (off_s + (staged ? size_t(P) * size_t(Ks) * esz : 0) + al) & ~al;
+    if (size_t(P) * size_t(staged ? Ks : K) * esz >= (size_t(1) << 32))
+      return fail(PQS_ERR_UNSUPPORTED, "QFIM too large");
+    int rc = ws_reserve(ctx, off_ps + (pad ? size_t(Kc) * esz : 0), st);
     if (rc) return rc;
     char *ws = (char *)ctx->ws;
     double *C = (double *)(ws + off_c);
     double2 *bv = (double2 *)(ws + off_b);
+    double2 *bpart = (double2 *)(ws + off_bp);
     int2 *d_tiles = (int2 *)(ws + off_t);
+    void *stage = ws + off_s;
     rc = upload(ctx, d_tiles, tl.data(), tl.size() * sizeof(int2), st);
     if (rc) return rc;
-    rc = gram_impl(ctx, dtype, psi, dim, 1, jac, dim, P, dim, bv, (double2 *)ws, st);
-    if (rc) return rc;
-    // the partials reuse the workspace head once the b vector is reduced
-    // (same stream, so in order)
-    if (dtype == PQS_C128)
-      hipLaunchKernelGGL(k_syrk_mfma<double>, dim3(unsigned(grid)), dim3(256), 0, st, (const double *)jac, K, P, K,
-                         kchunk, ks, d_tiles, T, (double *)ws);
-    else
-      hipLaunchKernelGGL(k_syrk_mfma<float>, dim3(unsigned(grid)), dim3(256), 0, st, (const float *)jac, K, P, K,
-                         kchunk, ks, d_tiles, T, (double *)ws);
-    HIPCHECK(hipGetLastError());
+    // column blocks after the first add to the partials (same stream, in
+    // order); the diagonal tiles' waves also form b = <J_p|psi>
+    for (int64_t c0 = 0; c0 < K; c0 += Kc) {
+      const void *src = (const char *)jac + size_t(c0) * esz;
+      const void *ps = (const char *)psi + size_t(c0) * esz;
+      int64_t ld = K, w = std::min(Kc, K - c0);
+      if (pad) {
+        const int64_t units = P * Ks;
+        void *pp = ws + off_ps;
+        if (dtype == PQS_C128) {
+          hipLaunchKernelGGL(k_pad_cols<double>, dim3(grid_for(units, 256)), dim3(256), 0, st, (const double *)src, K,
+                             (double *)stage, Ks, units);
+          hipLaunchKernelGGL(k_pad_cols<double>, dim3(1), dim3(256), 0, st, (const double *)ps, K, (double *)pp, Kc,
+                             Kc);
+        } else {
+          hipLaunchKernelGGL(k_pad_cols<float>, dim3(grid_for(units, 256)), dim3(256), 0, st, (const float *)src, K,
+                             (float *)stage, Ks, units);
+          hipLaunchKernelGGL(k_pad_cols<float>, dim3(1), dim3(256), 0, st, (const float *)ps, K, (float *)pp, Kc, Kc);
+        }
+        HIPCHECK(hipGetLastError());
+        src = stage;
+        ps = pp;
+        ld = Ks;
+        w = Kc;
+      } else if (staged) {
+        const int64_t units = P * (w * int64_t(esz) / 16);
+        hipLaunchKernelGGL(k_copy_cols, dim3(grid_for(units, 256)), dim3(256), 0, st, (const uint4 *)src,
+                           K * int64_t(esz) / 16, (uint4 *)stage, Ks * int64_t(esz) / 16, w * int64_t(esz) / 16, units);
+        HIPCHECK(hipGetLastError());
+        src = stage;
+        ld = Ks;
+      }
+      const int accum = c0 > 0;
+      if (dtype == PQS_C128)
+        hipLaunchKernelGGL(k_syrk_mfma<double>, dim3(unsigned(grid)), dim3(256), 0, st, (const double *)src, ld, P, w,
+                           kchunk, n_items, per_xcd, d_tiles, T, (const double *)ps, accum, (double *)ws, bpart);
+      else
+        hipLaunchKernelGGL(k_syrk_mfma<float>, dim3(unsigned(grid)), dim3(256), 0, st, (const float *)src, ld, P, w,
+                           kchunk, n_items, per_xcd, d_tiles, T, (const float *)ps, accum, (double *)ws, bpart);
+      HIPCHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_syrk_reduce, dim3(grid_for(P * P, 256)), dim3(256), 0, st, (const double *)ws, P, ks, C);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(P, 256)), dim3(256), 0, st, bpart, P, ks, bv);
     HIPCHECK(hipGetLastError());
     hipLaunchKernelGGL(k_qfim_combine_sym, dim3(grid_for(P * P, 256)), dim3(256), 0, st, C, bv, P, F);
     HIPCHECK(hipGetLastError());

@@ -339,6 +339,53 @@ def test_qfim_real_gram_paths(pyqc, n, p):
     assert np.allclose(F, F.T, atol=0)
 
 
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_qfim_few_amplitudes(pyqc, n):
+    """QFIMs of 1-3 qubit circuits (2-8 amplitudes: the product kernel's k
+    steps are 16 doubles, so 1-2 qubits take the zero-padded copy) against
+    the oracle; one R_y on |0> has F = 1."""
+    from pyramaterised_amd import engine as E
+    from pyramaterised_amd import templates as T
+    qc = pyqc.PQC(n)
+    for l in T.clifford_HE_layers(3, n):
+        qc.add_layer(l)
+    th = np.random.default_rng(40 + n).random(qc.n_true_params) * 2 * np.pi
+    qc.state = qc.run(list(th))
+    psi, J = qc.jacobian()
+    F = E.qfim(J, psi).cpu().numpy()
+    og = OC.clifford_he(3, n)
+    Fref = OM.qfim(OS.derivative_states(og, th, n), OS.run(og, th, n))
+    assert np.max(np.abs(F - Fref)) < 1e-12
+    q1 = pyqc.PQC(1)
+    q1.add_layer([pyqc.gates.R_y(0, 1)])
+    q1.state = q1.run([0.7])
+    psi1, J1 = q1.jacobian()
+    assert abs(E.qfim(J1, psi1).cpu().numpy()[0, 0] - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("N,P,dtype", [(20, 160, "complex64"), (19, 600, "complex128")])
+def test_qfim_column_blocks(pyqc, N, P, dtype):
+    """Jacobians above 1 GiB go through the product kernel in column blocks
+    copied to a compact buffer (three and ten blocks here, the last one
+    narrower): F = 4 Re(G - conj(b) b^T) against a float64 torch matmul of
+    the same random J and psi (rel-Frobenius 1e-11)."""
+    import torch
+    from pyramaterised_amd import engine as E
+    dev = E.device()
+    g = torch.Generator(device=dev).manual_seed(N)
+    cd = getattr(torch, dtype)
+    J = (torch.randn((P, 2 ** N), dtype=torch.complex128, device=dev, generator=g) / 2 ** (N / 2)).to(cd)
+    psi = (torch.randn(2 ** N, dtype=torch.complex128, device=dev, generator=g) / 2 ** (N / 2)).to(cd)
+    assert J.numel() * J.element_size() > 2 ** 30
+    F = E.qfim(J, psi)
+    Jd, pd = J.to(torch.complex128), psi.to(torch.complex128)
+    G = Jd @ Jd.conj().T
+    b = Jd.conj() @ pd
+    Fr = 4 * (G - torch.outer(b.conj(), b)).real
+    assert (torch.linalg.norm(F - Fr) / torch.linalg.norm(Fr)).item() < 1e-11
+    assert torch.equal(F, F.T)
+
+
 def test_config2_full_size_properties(pyqc):
     """The headline workload at its full size: 10,000 states of the 16-qubit,
     20-layer HEA (940 gates, P = 640) in one batched run.  Every state has
