@@ -1397,6 +1397,126 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SYWAVES))) 
   }
 }
 
+// Complex Gram G = A^dag B (A: m x K, B: nn x K complex, rows ld complex
+// apart) on the FP64 matrix cores, split over K.  One wave = one 64 x 32
+// tile of G (4 x 2 blocks of 16 x 16, real and imaginary accumulators: 128
+// registers) over one K chunk.  Lane l (row l % 16, k slot g = l / 16) loads
+// the complex amplitude kk + g of its row: Re G += Re a Re b + Im a Im b, Im G
+// += Re a Im b - Im a Re b, four products per block and step of 4
+// amplitudes.  Items, the XCD map and the load ring are those of the real
+// product above.
+constexpr int CGM = 64, CGN = 32, CGK = 4;
+
+template <typename TIn>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SYWAVES))) void k_cgram_mfma(
+    const TIn *__restrict__ A, int64_t lda, int64_t m, const TIn *__restrict__ B, int64_t ldb, int64_t nn, int64_t K,
+    int64_t kchunk, int64_t n_items, int64_t per_xcd, const int2 *__restrict__ tiles, int64_t T, int accum,
+    double2 *__restrict__ part) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, r16 = l & 15;
+  const int64_t slot = int64_t(blockIdx.x >> 3) * 4 + w;
+  if (slot >= per_xcd) return;
+  const int64_t mi = int64_t(blockIdx.x & 7) * per_xcd + slot;
+  if (mi >= n_items) return;
+  const int64_t chunk = mi / T;
+  const int2 tile = tiles[mi - chunk * T];
+  const int64_t i0 = int64_t(tile.x) * CGM, j0 = int64_t(tile.y) * CGN;
+  const int64_t k0 = chunk * kchunk;
+  if (k0 >= K) return;
+  const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  // byte offsets of the lane's rows (rows past the end re-read the last
+  // row: they feed only entries that are not stored)
+  constexpr int64_t cb = 2 * int64_t(sizeof(TIn));
+  uint32_t oa[4], ob[2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t r = i0 + 16 * a + r16;
+    oa[a] = uint32_t(((r < m ? r : m - 1) * lda + g) * cb);
+  }
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int64_t r = j0 + 16 * b + r16;
+    ob[b] = uint32_t(((r < nn ? r : nn - 1) * ldb + g) * cb);
+  }
+  d4_t re[4][2], im[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) re[a][b] = im[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
+  double2 X[SYRING][4], Y[SYRING][2];
+  auto load = [&](double2(&x)[4], double2(&y)[2], int64_t kk) {
+    const int64_t kl = kk < k1 ? kk : k0;
+    const char *ba = reinterpret_cast<const char *>(A + 2 * kl), *bb = reinterpret_cast<const char *>(B + 2 * kl);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) x[a] = syrk_ld2<TIn>(ba + oa[a]);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) y[b] = syrk_ld2<TIn>(bb + ob[b]);
+  };
+  auto mma = [&](const double2(&x)[4], const double2(&y)[2]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[a].x, y[b].x, re[a][b], 0, 0, 0);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[a].x, y[b].y, im[a][b], 0, 0, 0);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[a].y, y[b].y, re[a][b], 0, 0, 0);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-x[a].y, y[b].x, im[a][b], 0, 0, 0);
+  };
+#pragma unroll
+  for (int r = 0; r < SYRING; ++r) load(X[r], Y[r], k0 + r * CGK);
+  for (int64_t kk = k0; kk < k1; kk += SYRING * CGK) {
+#pragma unroll
+    for (int r = 0; r < SYRING; ++r) {
+      mma(X[r], Y[r]);
+      load(X[r], Y[r], kk + (r + SYRING) * CGK);
+    }
+  }
+  double2 *out = part + chunk * m * nn;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + 16 * a + 4 * r + g, j = j0 + 16 * b + r16;
+        if (i < m && j < nn) {
+          double2 v{re[a][b][r], im[a][b][r]};
+          if (accum) {
+            const double2 o = out[i * nn + j];
+            v.x += o.x;
+            v.y += o.y;
+          }
+          out[i * nn + j] = v;
+        }
+      }
+}
+
+// out[i][j] = sum over the split of part[z][i][j]; for A = B only the tiles
+// meeting the upper triangle were computed, out[i][j] = conj(out[j][i])
+// below it and the diagonal is real.
+__global__ void k_cgram_reduce(const double2 *__restrict__ part, int64_t m, int64_t nn, int64_t ks, int sym,
+                               double2 *__restrict__ out) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m * nn) return;
+  const int64_t i = e / nn, j = e - i * nn;
+  const bool lower = sym && i > j;
+  const int64_t src = lower ? j * nn + i : e;
+  double2 a{0, 0};
+  for (int64_t z = 0; z < ks; ++z) {
+    a.x += part[z * m * nn + src].x;
+    a.y += part[z * m * nn + src].y;
+  }
+  // <a_i|a_i> is real: its products cancel term by term (as in vdot)
+  out[e] = lower ? double2{a.x, -a.y} : sym && i == j ? double2{a.x, 0.0} : a;
+}
+
 // dst[r][0, w) = src[r][0, w) for P rows (16-byte units, row strides lds and
 // ldd): a column block of the Jacobian into a compact buffer.
 __global__ void k_copy_cols(const uint4 *__restrict__ src, int64_t lds, uint4 *__restrict__ dst, int64_t ldd,
@@ -2727,6 +2847,102 @@ int gram_impl(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m
   return PQS_OK;
 }
 
+// A^dag B on the matrix cores (k_cgram_mfma).  Returns PQS_ERR_UNSUPPORTED,
+// without touching the stream, for shapes it does not take: a K that is not
+// a whole number of load-ring steps (fewer than 8 amplitudes) or operands
+// whose rows cannot be addressed in 32 bits nor copied in 16-byte units.
+int cgram_mfma(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m, const void *b, int64_t ldb,
+               int64_t nn, int64_t dim, double2 *out, hipStream_t st) {
+  const int64_t cb = dtype == PQS_C128 ? 16 : 8;  // bytes per amplitude
+  const int64_t ring = SYRING * CGK;
+  if (dim % ring != 0) return PQS_ERR_UNSUPPORTED;
+  const bool sym = a == b && lda == ldb && m == nn;
+  // operands above 1 GiB in column blocks of at most 512 MiB copied to a
+  // buffer with padded rows (as for the QFIM: TLB reach and L2 channels)
+  const size_t rows = size_t(m) + (sym ? 0 : size_t(nn));
+  const bool staged = rows * size_t(dim) * size_t(cb) > kQfimStageBytes;
+  int64_t Kc = dim;
+  if (staged) {
+    if ((lda * cb) % 16 || (ldb * cb) % 16) return PQS_ERR_UNSUPPORTED;
+    Kc = int64_t(kQfimBlockBytes / (rows * size_t(cb))) / ring * ring;
+    Kc = std::max<int64_t>(ring, std::min<int64_t>(Kc, dim));
+  }
+  const int64_t Ks = Kc + 16;
+  const int64_t la = staged ? Ks : lda, lb = staged ? Ks : ldb;
+  if (size_t(m) * size_t(la) * size_t(cb) >= (size_t(1) << 32) || size_t(nn) * size_t(lb) * size_t(cb) >= (size_t(1) << 32))
+    return PQS_ERR_UNSUPPORTED;
+  const int64_t tm = (m + CGM - 1) / CGM, tn = (nn + CGN - 1) / CGN;
+  std::vector<int2> tl;
+  for (int64_t x = 0; x < tm; ++x)
+    for (int64_t y = 0; y < tn; ++y)
+      if (!sym || CGM * x <= CGN * y + CGN - 1) tl.push_back(int2{int(x), int(y)});
+  const int64_t T = int64_t(tl.size());
+  // the K split that leaves the fewest wave slots idle over whole rounds
+  // (chunks of at least 256 amplitudes, partials within 2 GiB)
+  const int64_t slots = 4 * SYWAVES * int64_t(ctx->n_cu);
+  const int64_t ks_max = std::max<int64_t>(
+      1, std::min<int64_t>({Kc / 256, int64_t(64), int64_t((size_t(2) << 30) / (size_t(m) * size_t(nn) * 16))}));
+  int64_t ks = 1;
+  double best = -1;
+  for (int64_t k = 1; k <= ks_max; ++k) {
+    const int64_t items = T * k, rounds = (items + slots - 1) / slots;
+    const double fill = double(items) / double(rounds * slots);
+    if (fill > best + 0.01) {
+      best = fill;
+      ks = k;
+    }
+  }
+  int64_t kchunk = (Kc + ks - 1) / ks;
+  kchunk = (kchunk + ring - 1) / ring * ring;
+  ks = (Kc + kchunk - 1) / kchunk;
+  const int64_t n_items = T * ks, per_xcd = (n_items + 7) / 8;
+  const int64_t grid = 8 * ((per_xcd + 3) / 4);
+  if (grid > 0x7fffffff) return PQS_ERR_UNSUPPORTED;
+  const size_t al = 255;
+  const size_t off_t = (size_t(ks) * size_t(m) * size_t(nn) * sizeof(double2) + al) & ~al;
+  const size_t off_a = (off_t + tl.size() * sizeof(int2) + al) & ~al;
+  const size_t off_b = (off_a + (staged ? size_t(m) * size_t(Ks) * size_t(cb) : 0) + al) & ~al;
+  const size_t end = off_b + (staged && !sym ? size_t(nn) * size_t(Ks) * size_t(cb) : 0);
+  int rc = ws_reserve(ctx, end, st);
+  if (rc) return rc;
+  char *ws = (char *)ctx->ws;
+  double2 *part = (double2 *)ws;
+  int2 *d_tiles = (int2 *)(ws + off_t);
+  rc = upload(ctx, d_tiles, tl.data(), tl.size() * sizeof(int2), st);
+  if (rc) return rc;
+  for (int64_t c0 = 0; c0 < dim; c0 += Kc) {
+    const int64_t w = std::min(Kc, dim - c0);
+    const void *pa = (const char *)a + size_t(c0) * size_t(cb), *pb = (const char *)b + size_t(c0) * size_t(cb);
+    if (staged) {
+      const int64_t wu = w * cb / 16;
+      hipLaunchKernelGGL(k_copy_cols, dim3(grid_for(m * wu, 256)), dim3(256), 0, st, (const uint4 *)pa,
+                         lda * cb / 16, (uint4 *)(ws + off_a), Ks * cb / 16, wu, m * wu);
+      HIPCHECK(hipGetLastError());
+      pa = ws + off_a;
+      if (sym) {
+        pb = pa;
+      } else {
+        hipLaunchKernelGGL(k_copy_cols, dim3(grid_for(nn * wu, 256)), dim3(256), 0, st, (const uint4 *)pb,
+                           ldb * cb / 16, (uint4 *)(ws + off_b), Ks * cb / 16, wu, nn * wu);
+        HIPCHECK(hipGetLastError());
+        pb = ws + off_b;
+      }
+    }
+    const int accum = c0 > 0;
+    if (dtype == PQS_C128)
+      hipLaunchKernelGGL(k_cgram_mfma<double>, dim3(unsigned(grid)), dim3(256), 0, st, (const double *)pa, la, m,
+                         (const double *)pb, lb, nn, w, kchunk, n_items, per_xcd, d_tiles, T, accum, part);
+    else
+      hipLaunchKernelGGL(k_cgram_mfma<float>, dim3(unsigned(grid)), dim3(256), 0, st, (const float *)pa, la, m,
+                         (const float *)pb, lb, nn, w, kchunk, n_items, per_xcd, d_tiles, T, accum, part);
+    HIPCHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_cgram_reduce, dim3(grid_for(m * nn, 256)), dim3(256), 0, st, (const double2 *)part, m, nn, ks,
+                     int(sym), out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2737,6 +2953,10 @@ int pqs_gram(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m,
   if (m <= 0 || nn <= 0 || dim <= 0 || lda < dim || ldb < dim) return fail(PQS_ERR_ARG, "bad sizes");
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
+  if (m > 1) {
+    const int rc = cgram_mfma(ctx, dtype, a, lda, m, b, ldb, nn, dim, (double2 *)out, st);
+    if (rc != PQS_ERR_UNSUPPORTED) return rc;  // else: shapes the matrix-core path does not take
+  }
   int rc = ws_reserve(ctx, gram_part_bytes(ctx, m, nn, dim), st);
   if (rc) return rc;
   return gram_impl(ctx, dtype, a, lda, m, b, ldb, nn, dim, (double2 *)out, (double2 *)ctx->ws, st);
