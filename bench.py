@@ -286,6 +286,50 @@ def speedups_16q(pyqc, args, head):
             "qfim16_plain_resim_s": t_resim, "qfim16_overlap_call_us": t_ov * 1e6, "jac_forward_vs_plain_maxdiff": jdiff}
 
 
+def events_ms(f, reps=5):
+    """Best of ``reps`` HIP-event timings of f() on the current stream
+    (after one untimed call)."""
+    import torch
+    f()
+    sync()
+    best = 1e30
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        f()
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b))
+    return best
+
+
+def leg_gram(pyqc):
+    """The Gram steps alone on the headline circuit (16 qubits, P = 640): the
+    QFIM from its Jacobian (real product on the matrix cores, b vector folded
+    in) and the all-pairs fidelity Gram of 2000 sample states (complex
+    product, upper triangle); HIP events, FP64 fractions of the upper
+    triangle's flops."""
+    import torch
+    from pyramaterised_amd import engine as E
+    qc = build_c2(pyqc)
+    P, n = qc.n_true_params, qc.n_qubits
+    rng = np.random.default_rng(16)
+    qc.set_params(list(rng.random(P) * 2 * np.pi))
+    psi, J = qc.jacobian()
+    gram_ms = events_ms(lambda: E.qfim(J, psi))
+    gram_flops = 2.0 * (P * (P + 1) / 2) * 2 * 2 ** n
+    del J, psi
+    torch.cuda.empty_cache()
+    S2 = 2000
+    sts = qc.run_batch(rng.random((S2, P)) * 2 * np.pi)
+    ap_ms = events_ms(lambda: E.gram(sts, sts))
+    ap_flops = 8.0 * (S2 * (S2 + 1) / 2) * 2 ** n
+    del sts
+    torch.cuda.empty_cache()
+    return {"qfim16_gram_ms": gram_ms, "qfim16_gram_fp64_frac": gram_flops / (gram_ms * 1e-3) / 78.6e12,
+            "allpairs2000_gram_ms": ap_ms, "allpairs2000_gram_fp64_frac": ap_flops / (ap_ms * 1e-3) / 78.6e12}
+
+
 def leg_adjoint20(pyqc):
     """Energy gradient of a 20-qubit, 40-layer HEA (P = 1600, H = Z0 Z1):
     the adjoint pass (two states) against the forward-Jacobian route (P
@@ -328,6 +372,15 @@ def leg_adjoint20(pyqc):
     out["max_abs_diff"] = float(np.max(np.abs(out.pop("adjoint_grad") - out.pop("jacobian_grad"))))
     out["speedup"] = out["jacobian_ms"] / out["adjoint_ms"]
     out["measurement_points"] = len(adjoint.program(qc).steps)
+    # the QFIM of the same circuit: the P = 1600 Gram of a 26.8 GB Jacobian
+    # (column blocks through the matrix cores), HIP events around pqs_qfim
+    psi, J = qc.jacobian()
+    ms = events_ms(lambda: E.qfim(J, psi), reps=3)
+    flops = 2.0 * (P * (P + 1) / 2) * 2 * 2 ** n  # upper triangle of Re(J J^dag)
+    out["qfim_gram_ms"] = ms
+    out["qfim_gram_fp64_frac"] = flops / (ms * 1e-3) / 78.6e12
+    del J, psi
+    torch.cuda.empty_cache()
     return out
 
 
@@ -419,8 +472,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-rate", type=float, default=25.0)
     ap.add_argument("--quick", action="store_true", help="headline only")
-    ap.add_argument("--legs", default="28q,qfim12,adjoint20,serving,speedups",
-                    help="extra legs after the headline (comma list of 28q, qfim12, adjoint20, serving, speedups)")
+    ap.add_argument("--legs", default="28q,qfim12,adjoint20,gram,serving,speedups",
+                    help="extra legs after the headline (comma list of 28q, qfim12, adjoint20, gram, serving, "
+                         "speedups)")
     ap.add_argument("--only-legs", action="store_true",
                     help="skip the headline and the CPU baseline; print the legs' JSON only (for profiling a leg)")
     ap.add_argument("--shape", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
@@ -467,6 +521,10 @@ def main():
             log("20-qubit energy gradient: adjoint pass vs forward Jacobian")
             extra["gradient_20q_adjoint"] = leg_adjoint20(pyqc)
             log(json.dumps(extra["gradient_20q_adjoint"]))
+        if "gram" in legs:
+            log("16-qubit Gram steps: QFIM (P = 640) and all-pairs fidelities (2000 states)")
+            extra["gram_16q"] = leg_gram(pyqc)
+            log(json.dumps(extra["gram_16q"]))
         if "serving" in legs:
             log(f"serving mix: {args.serve_requests} requests at {args.serve_rate}/s")
             extra["serving"] = leg_serving(args)
