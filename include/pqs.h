@@ -154,6 +154,15 @@ typedef struct pqs_plan pqs_plan;
 int pqs_abi_version(void);
 const char *pqs_last_error(void);
 
+/* Named ranges for profilers (rocprofv3 --marker-trace).  With the
+ * environment variable PQS_ROCTX=1 the library loads the rocprofiler-sdk
+ * roctx library (or libroctx64) and marks
+ * its compute entry points and the caller's ranges; otherwise these calls do
+ * nothing.  Ranges nest per host thread.  No reference counterpart: the
+ * serving loop marks its batches with them. */
+int pqs_range_push(const char *name);
+int pqs_range_pop(void);
+
 /* Context on one device; owns a workspace that grows on demand. */
 int pqs_ctx_create(int device, pqs_ctx **out);
 int pqs_ctx_destroy(pqs_ctx *ctx);

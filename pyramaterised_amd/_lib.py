@@ -47,6 +47,8 @@ vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.c_void_p
 SIGNATURES = {
     "pqs_abi_version": (C.c_int, []),
     "pqs_last_error": (C.c_char_p, []),
+    "pqs_range_push": (C.c_int, [C.c_char_p]),
+    "pqs_range_pop": (C.c_int, []),
     "pqs_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
     "pqs_ctx_destroy": (C.c_int, [vp]),
     "pqs_device_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.c_char_p, i32]),

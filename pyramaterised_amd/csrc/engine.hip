@@ -17,6 +17,7 @@
 // else runs in the same batch.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -60,6 +61,43 @@ int hip_fail(hipError_t e, const char *what) {
     hipError_t e_ = (x);                              \
     if (e_ != hipSuccess) return hip_fail(e_, #x);    \
   } while (0)
+
+// roctx ranges around the entry points (seen by rocprofv3 --marker-trace):
+// with PQS_ROCTX=1 the library loads the rocprofiler-sdk roctx library (the
+// one rocprofv3 intercepts; the older libroctx64 otherwise) at first use;
+// without it the ranges cost one branch.
+struct Roctx {
+  int (*push)(const char *) = nullptr;
+  int (*pop)() = nullptr;
+};
+const Roctx &roctx() {
+  static const Roctx r = [] {
+    Roctx x;
+    const char *e = std::getenv("PQS_ROCTX");
+    if (e && e[0] == '1') {
+      void *h = nullptr;
+      for (const char *lib : {"librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1",
+                              "libroctx64.so", "/opt/rocm/lib/libroctx64.so"})
+        if (!h) h = dlopen(lib, RTLD_NOW | RTLD_LOCAL);
+      if (h) {
+        x.push = reinterpret_cast<int (*)(const char *)>(dlsym(h, "roctxRangePushA"));
+        x.pop = reinterpret_cast<int (*)()>(dlsym(h, "roctxRangePop"));
+        if (!x.push || !x.pop) x.push = nullptr, x.pop = nullptr;
+      }
+    }
+    return x;
+  }();
+  return r;
+}
+struct Range {
+  const bool on;
+  explicit Range(const char *name) : on(roctx().push != nullptr) {
+    if (on) roctx().push(name);
+  }
+  ~Range() {
+    if (on) roctx().pop();
+  }
+};
 
 // Instance record as the sweep kernel reads it.
 struct InstDev {
@@ -2098,6 +2136,17 @@ size_t amp_bytes(int dtype) { return dtype == PQS_C128 ? 16 : 8; }
 
 extern "C" {
 
+int pqs_range_push(const char *name) {
+  if (!name) return fail(PQS_ERR_ARG, "NULL range name");
+  if (roctx().push) roctx().push(name);
+  return PQS_OK;
+}
+
+int pqs_range_pop(void) {
+  if (roctx().pop) roctx().pop();
+  return PQS_OK;
+}
+
 int pqs_ctx_create(int device, pqs_ctx **out) {
   if (!out) return fail(PQS_ERR_ARG, "out is NULL");
   *out = nullptr;
@@ -2301,6 +2350,7 @@ static int run_instances(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, con
 
 int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst, const void *init_tab,
             void *stream) {
+  Range rg_("pqs_run");
   if (!ctx || !p) return fail(PQS_ERR_ARG, "NULL ctx or plan");
   if (n_inst < 0 || (n_inst > 0 && !insts)) return fail(PQS_ERR_ARG, "bad instance array");
   if (n_inst == 0) return PQS_OK;
@@ -2332,6 +2382,7 @@ int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst
 int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride, int64_t batch,
                   const double *params, int64_t param_stride, const int32_t *tangents, int32_t init,
                   const void *src, const void *init_tab, void *stream) {
+  Range rg_("pqs_run_batch");
   if (!ctx || !p || !states) return fail(PQS_ERR_ARG, "NULL ctx, plan or states");
   if (batch < 0) return fail(PQS_ERR_ARG, "negative batch");
   if (batch == 0) return PQS_OK;
@@ -2362,6 +2413,7 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride,
 
 int pqs_run_multi(pqs_ctx *ctx, pqs_plan *const *plans, int32_t n_plans, const pqs_instance *insts,
                   const int32_t *inst_plan, const void *const *init_tabs, int64_t n_inst, void *stream) {
+  Range rg_("pqs_run_multi");
   if (!ctx || !plans || n_plans <= 0) return fail(PQS_ERR_ARG, "NULL ctx or no plans");
   if (n_inst < 0 || (n_inst > 0 && (!insts || !inst_plan))) return fail(PQS_ERR_ARG, "bad instance arrays");
   if (n_inst == 0) return PQS_OK;
@@ -2559,6 +2611,7 @@ int pqs_batch_table(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_strid
 
 int pqs_run_table(pqs_ctx *ctx, pqs_plan *p, const void *table, int64_t batch, const void *init_tab,
                   void *stream) {
+  Range rg_("pqs_run_table");
   if (!ctx || !p || !table) return fail(PQS_ERR_ARG, "NULL ctx, plan or table");
   if (batch <= 0) return fail(PQS_ERR_ARG, "batch must be positive");
   if (!p->d_ops || p->dev_device != ctx->device) return fail(PQS_ERR_ARG, "plan not on this device: table it first");
@@ -2574,6 +2627,7 @@ int pqs_run_table(pqs_ctx *ctx, pqs_plan *p, const void *table, int64_t batch, c
 int pqs_run_batch_noalloc(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride, int64_t batch,
                           const double *params, int64_t param_stride, int32_t init, const void *src, void *alt,
                           const void *init_tab, void *stream) {
+  Range rg_("pqs_run_batch_noalloc");
   if (!ctx || !p || !states) return fail(PQS_ERR_ARG, "NULL ctx, plan or states");
   if (batch < 0) return fail(PQS_ERR_ARG, "negative batch");
   if (batch > kNoallocMaxBatch)
@@ -2606,6 +2660,7 @@ int pqs_run_batch_noalloc(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state
 
 int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, const void *src,
                  const void *init_tab, void *psi, void *jac, void *stream) {
+  Range rg_("pqs_jacobian");
   if (!ctx || !p || !psi) return fail(PQS_ERR_ARG, "NULL ctx, plan or psi");
   if (p->n_tangents > 0 && !jac) return fail(PQS_ERR_ARG, "NULL jacobian buffer");
   if (init < 0 || init > 2) return fail(PQS_ERR_ARG, "init must be 0, 1 or 2");
@@ -2732,6 +2787,7 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
 
 int pqs_inner(pqs_ctx *ctx, int32_t dtype, const void *base, int64_t dim, const int64_t *a_off,
               const int64_t *b_off, int64_t n_pairs, double *out, void *stream) {
+  Range rg_("pqs_inner");
   if (!ctx || !base || !a_off || !b_off || !out) return fail(PQS_ERR_ARG, "NULL argument");
   if (dim <= 0 || n_pairs < 0) return fail(PQS_ERR_ARG, "bad sizes");
   if (n_pairs == 0) return PQS_OK;
@@ -2757,6 +2813,7 @@ int pqs_inner(pqs_ctx *ctx, int32_t dtype, const void *base, int64_t dim, const 
 
 int pqs_mw(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t batch, double *rho, double *q,
            void *stream) {
+  Range rg_("pqs_mw");
   if (!ctx || !states) return fail(PQS_ERR_ARG, "NULL argument");
   if (n < 1 || n > pqs::kMaxQubits || batch < 0) return fail(PQS_ERR_ARG, "bad sizes");
   if (batch == 0) return PQS_OK;
@@ -2949,6 +3006,7 @@ extern "C" {
 
 int pqs_gram(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m, const void *b, int64_t ldb,
              int64_t nn, int64_t dim, double *out, void *stream) {
+  Range rg_("pqs_gram");
   if (!ctx || !a || !b || !out) return fail(PQS_ERR_ARG, "NULL argument");
   if (m <= 0 || nn <= 0 || dim <= 0 || lda < dim || ldb < dim) return fail(PQS_ERR_ARG, "bad sizes");
   DeviceGuard g(ctx->device);
@@ -2964,6 +3022,7 @@ int pqs_gram(pqs_ctx *ctx, int32_t dtype, const void *a, int64_t lda, int64_t m,
 
 int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac, int64_t P, const void *psi, int64_t dim, double *F,
              void *stream) {
+  Range rg_("pqs_qfim");
   if (!ctx || !jac || !psi || !F) return fail(PQS_ERR_ARG, "NULL argument");
   if (P <= 0 || dim <= 0) return fail(PQS_ERR_ARG, "bad sizes");
   DeviceGuard g(ctx->device);
@@ -3109,6 +3168,7 @@ static std::vector<PauliTerm> pauli_terms(int n, const uint64_t *xm, const uint6
 
 int pqs_apply_pauli(pqs_ctx *ctx, int32_t dtype, const void *x, int32_t n, const uint64_t *xm, const uint64_t *zm,
                     const double *coef, int32_t nt, void *out, void *stream) {
+  Range rg_("pqs_apply_pauli");
   if (!ctx || !x || !out || (nt > 0 && (!xm || !zm || !coef))) return fail(PQS_ERR_ARG, "NULL argument");
   if (n < 1 || n > pqs::kMaxQubits || nt < 0) return fail(PQS_ERR_ARG, "bad sizes");
   if (x == out) return fail(PQS_ERR_ARG, "out must not be the input state");
@@ -3134,6 +3194,7 @@ int pqs_apply_pauli(pqs_ctx *ctx, int32_t dtype, const void *x, int32_t n, const
 int pqs_pauli_transition(pqs_ctx *ctx, int32_t dtype, const void *bra, const void *ket, int32_t n,
                          const uint64_t *xm, const uint64_t *zm, const double *coef, int32_t nt, double *out,
                          void *stream) {
+  Range rg_("pqs_pauli_transition");
   if (!ctx || !bra || !ket || !out || (nt > 0 && (!xm || !zm || !coef))) return fail(PQS_ERR_ARG, "NULL argument");
   if (n < 1 || n > pqs::kMaxQubits || nt < 0) return fail(PQS_ERR_ARG, "bad sizes");
   if (nt == 0) return PQS_OK;
@@ -3167,6 +3228,7 @@ int pqs_pauli_transition(pqs_ctx *ctx, int32_t dtype, const void *bra, const voi
 
 int pqs_expect_pauli(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t batch, const uint64_t *xm,
                      const uint64_t *zm, const double *coef, int32_t nt, double *out, void *stream) {
+  Range rg_("pqs_expect_pauli");
   if (!ctx || !states || !out || (nt > 0 && (!xm || !zm || !coef))) return fail(PQS_ERR_ARG, "NULL argument");
   if (n < 1 || n > pqs::kMaxQubits || batch < 0 || nt < 0) return fail(PQS_ERR_ARG, "bad sizes");
   if (batch == 0) return PQS_OK;
@@ -3234,6 +3296,7 @@ int pqs_histogram(pqs_ctx *ctx, const double *values, int64_t n, const double *e
 
 int pqs_apply_dense(pqs_ctx *ctx, int32_t dtype, const double *u, int64_t dim, const void *states,
                     int64_t stride, int64_t batch, void *out, void *stream) {
+  Range rg_("pqs_apply_dense");
   if (!ctx || !u || !states || !out) return fail(PQS_ERR_ARG, "NULL argument");
   if (dim <= 0 || batch < 0 || stride < dim) return fail(PQS_ERR_ARG, "bad sizes");
   if (dim > (int64_t(1) << 14)) return fail(PQS_ERR_UNSUPPORTED, "dense operators are limited to 14 qubits");
@@ -3264,6 +3327,7 @@ int pqs_abs2(pqs_ctx *ctx, const double *z, int64_t n, double *out, void *stream
 
 int pqs_magic(pqs_ctx *ctx, int32_t dtype, const void *states, int32_t n, int64_t batch, double alpha,
               double *out, void *stream) {
+  Range rg_("pqs_magic");
   if (!ctx || !states || !out) return fail(PQS_ERR_ARG, "NULL argument");
   if (n < 1 || batch < 0) return fail(PQS_ERR_ARG, "bad sizes");
   if (n > 13) return fail(PQS_ERR_UNSUPPORTED, "stabiliser entropy is limited to 13 qubits (4^n work)");

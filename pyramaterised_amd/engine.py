@@ -95,6 +95,18 @@ def using(ctx, stream):
         _tls.override = prev
 
 
+@contextlib.contextmanager
+def marked(name):
+    """A named profiler range (roctx, seen by rocprofv3 --marker-trace when
+    PQS_ROCTX=1) around the block; nothing otherwise."""
+    lib = L.lib()
+    check(lib.pqs_range_push(str(name).encode()))
+    try:
+        yield
+    finally:
+        lib.pqs_range_pop()
+
+
 _CHILD = r"""
 import ctypes, sys
 lib = ctypes.CDLL(sys.argv[1])

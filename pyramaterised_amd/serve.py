@@ -60,18 +60,26 @@ class Request:
         self.params, self.arrival = params, float(arrival)
 
 
+OK, CANCELLED, ERROR = "ok", "cancelled", "error"
+
+
 class Result:
     """Outcome of a request.  ``state`` is a device tensor view (resident
     until ``Server.release``); ``Q`` the Meyer-Wallach value; ``qfim`` a
-    (P, P) numpy array.  Times are seconds on the server's clock."""
+    (P, P) numpy array.  ``status``: None while queued or running, then
+    "ok", "cancelled" (``Server.cancel``) or "error" (its launch raised;
+    ``error`` holds the message).  Times are seconds on the server's
+    clock."""
 
-    __slots__ = ("rid", "kind", "n", "depth", "state", "Q", "qfim", "arrival", "launched", "done")
+    __slots__ = ("rid", "kind", "n", "depth", "state", "Q", "qfim", "arrival", "launched", "done", "status",
+                 "error")
 
     def __init__(self, req):
         self.rid, self.kind, self.n, self.depth = req.rid, req.kind, req.n, req.depth
         self.state = self.Q = self.qfim = None
         self.arrival = req.arrival
         self.launched = self.done = None
+        self.status = self.error = None
 
     @property
     def latency(self):
@@ -113,7 +121,9 @@ def result_to_json(res, with_state=False):
     float64 array, the state (copied to the host) only with ``with_state``,
     and the server's times."""
     d = {"rid": res.rid, "kind": res.kind, "n": res.n, "depth": res.depth, "arrival": res.arrival,
-         "launched": res.launched, "done": res.done, "Q": res.Q}
+         "launched": res.launched, "done": res.done, "Q": res.Q, "status": res.status}
+    if res.error is not None:
+        d["error"] = res.error
     if res.qfim is not None:
         d["qfim"] = _pack_array(np.asarray(res.qfim, dtype=np.float64))
     if with_state and res.state is not None:
@@ -275,6 +285,8 @@ class Server:
         self._order = []          # class keys, oldest head-of-queue first
         self._order_dirty = False
         self._jscratch = {}       # slot -> reusable Jacobian buffer (stream-ordered reuse)
+        self._cancel_running = set()  # rids cancelled while their batch runs
+        self._n_settled = 0       # requests cancelled in the queue or failed at launch, not yet counted
 
     # ---------------------------------------------------------------- sizing
     def request_bytes(self, req):
@@ -327,6 +339,50 @@ class Server:
         if group[0] == 0:
             self.in_use -= group[1]
             del self._groups[h]
+
+    def cancel(self, rid, now=None):
+        """Cancel a request.  Queued: it leaves its queue at once.  Running:
+        its batch completes (other requests of the batch are unaffected)
+        and its result is dropped when the batch retires.  Finished: its
+        resident state is released.  Returns the result's status after the
+        call ("cancelled", or "ok" / "error" for a finished request).  Call
+        it from the thread that drives the server (between ``step`` calls
+        or from ``run``'s progress callback)."""
+        res = self.results.get(rid)
+        if res is None:
+            raise KeyError(f"unknown request {rid}")
+        if res.status is not None:
+            if res.status == OK:
+                self.release(rid)
+            return res.status
+        if res.launched is None:
+            key = (res.n, res.depth, res.kind)
+            q = self.queues.get(key)
+            for r in q:
+                if r.rid == rid:
+                    q.remove(r)
+                    break
+            self._n_pending -= 1
+            self._order_dirty = True
+            res.status, res.done = CANCELLED, now
+            self._n_settled += 1
+            return CANCELLED
+        self._cancel_running.add(rid)
+        return CANCELLED
+
+    def _settle(self):
+        n, self._n_settled = self._n_settled, 0
+        return n
+
+    def _fail(self, reqs, need, err, now):
+        """A launch raised before any work was queued: its requests end with
+        status "error" and give their bytes back; the server goes on."""
+        self.in_use -= need
+        for r in reqs:
+            res = self.results[r.rid]
+            res.status, res.error, res.done = ERROR, f"{type(err).__name__}: {err}", now
+            self._cancel_running.discard(r.rid)
+        self._n_settled += len(reqs)
 
     # ---------------------------------------------------------------- loop
     def _free_slot(self, large):
@@ -439,25 +495,30 @@ class Server:
         b = _Batch()
         b.reqs, b.bytes, b.slot = reqs, need, slot
         b.out = b.q = b.F = None
-        with E.using(slot.ctx, slot.stream):
-            if kind == STATE_MW:
-                params = _to_device(np.stack([r.params for r in reqs]), self.dev)
-                b.out = E.run(c.best("plan"), params, c.init)
-                _, b.q = E.mw(b.out, n)
-            else:
-                r = reqs[0]
-                P, dim = 2 * r.n * r.depth, 1 << r.n
-                buf = self._jscratch.get(id(slot))
-                if buf is None or buf.numel() < P * dim:
-                    # grown, never shrunk: launches on one stream reuse it in order
-                    buf = self._jscratch[id(slot)] = torch.empty(P * dim, dtype=E.CDTYPE[self.dtype], device=self.dev)
-                J = buf[:P * dim].view(P, dim)
-                psi, J = E.jacobian(c.best("gplan"), _to_device(r.params, self.dev), c.init, jac_out=J)
-                b.F = E.qfim(J, psi)
-                b.out = psi
-                del J
-            b.event = torch.cuda.Event()
-            b.event.record(slot.stream)
+        try:
+            with E.using(slot.ctx, slot.stream), E.marked(f"serve {kind} n={n} depth={depth} batch={len(reqs)}"):
+                if kind == STATE_MW:
+                    params = _to_device(np.stack([r.params for r in reqs]), self.dev)
+                    b.out = E.run(c.best("plan"), params, c.init)
+                    _, b.q = E.mw(b.out, n)
+                else:
+                    r = reqs[0]
+                    P, dim = 2 * r.n * r.depth, 1 << r.n
+                    buf = self._jscratch.get(id(slot))
+                    if buf is None or buf.numel() < P * dim:
+                        # grown, never shrunk: launches on one stream reuse it in order
+                        buf = self._jscratch[id(slot)] = torch.empty(P * dim, dtype=E.CDTYPE[self.dtype],
+                                                                     device=self.dev)
+                    J = buf[:P * dim].view(P, dim)
+                    psi, J = E.jacobian(c.best("gplan"), _to_device(r.params, self.dev), c.init, jac_out=J)
+                    b.F = E.qfim(J, psi)
+                    b.out = psi
+                    del J
+                b.event = torch.cuda.Event()
+                b.event.record(slot.stream)
+        except Exception as err:  # a launch-time failure ends this batch only
+            self._fail(reqs, need, err, now)
+            return
         for r in reqs:
             self.results[r.rid].launched = now
         slot.inflight.append(b)
@@ -480,20 +541,24 @@ class Server:
         b = _Batch()
         b.reqs, b.bytes, b.slot = order, need, slot
         b.F = None
-        with E.using(slot.ctx, slot.stream):
-            jobs = []
-            for (n, depth), rs in groups.items():
-                c = self.circuit(n, depth)
-                jobs.append((c.plan, _to_device(np.stack([r.params for r in rs]), self.dev), c.init))
-            outs = E.run_multi(jobs)
-            qs = []
-            b.out = []
-            for ((n, _), rs), out in zip(groups.items(), outs):
-                qs.append(E.mw(out, n)[1])
-                b.out += [out[i] for i in range(len(rs))]
-            b.q = torch.cat(qs)
-            b.event = torch.cuda.Event()
-            b.event.record(slot.stream)
+        try:
+            with E.using(slot.ctx, slot.stream), E.marked(f"serve merged classes={len(groups)} batch={len(order)}"):
+                jobs = []
+                for (n, depth), rs in groups.items():
+                    c = self.circuit(n, depth)
+                    jobs.append((c.plan, _to_device(np.stack([r.params for r in rs]), self.dev), c.init))
+                outs = E.run_multi(jobs)
+                qs = []
+                b.out = []
+                for ((n, _), rs), out in zip(groups.items(), outs):
+                    qs.append(E.mw(out, n)[1])
+                    b.out += [out[i] for i in range(len(rs))]
+                b.q = torch.cat(qs)
+                b.event = torch.cuda.Event()
+                b.event.record(slot.stream)
+        except Exception as err:  # a launch-time failure ends this batch only
+            self._fail(order, need, err, now)
+            return
         for r in order:
             self.results[r.rid].launched = now
         slot.inflight.append(b)
@@ -510,36 +575,56 @@ class Server:
                 kind = b.reqs[0].kind
                 group = self._next_group
                 self._next_group += 1
-                held_total = 0
+                held_total, n_held = 0, 0
                 if kind == STATE_MW:
                     qs = b.q.cpu().numpy()
                     for i, r in enumerate(b.reqs):
                         res = self.results[r.rid]
-                        res.Q = float(qs[i])
                         res.done = now
+                        cancelled = r.rid in self._cancel_running
+                        if cancelled:
+                            self._cancel_running.discard(r.rid)
+                            res.status = CANCELLED
+                        else:
+                            res.Q = float(qs[i])
+                            res.status = OK
                         if self.keep_states:
-                            res.state = b.out[i]
+                            # a cancelled row stays part of the batch tensor:
+                            # its bytes return with the group's last row
                             held = self.amp * (1 << r.n)
-                            self._held[r.rid] = group
                             held_total += held
                             self.in_use -= self.request_bytes(r) - held
+                            if not cancelled:
+                                res.state = b.out[i]
+                                self._held[r.rid] = group
+                                n_held += 1
                         else:
                             self.in_use -= self.request_bytes(r)
+                elif b.reqs[0].rid in self._cancel_running:
+                    r = b.reqs[0]
+                    self._cancel_running.discard(r.rid)
+                    res = self.results[r.rid]
+                    res.status, res.done = CANCELLED, now
+                    self.in_use -= self.request_bytes(r)
                 else:
                     r = b.reqs[0]
                     res = self.results[r.rid]
                     res.qfim = b.F.cpu().numpy()
                     res.done = now
+                    res.status = OK
                     if self.keep_states:
                         res.state = b.out
                         held = self.amp * (1 << r.n)
                         self._held[r.rid] = group
                         held_total += held
+                        n_held += 1
                         self.in_use -= self.request_bytes(r) - held
                     else:
                         self.in_use -= self.request_bytes(r)
-                if held_total:
-                    self._groups[group] = [len(b.reqs), held_total]
+                if n_held:
+                    self._groups[group] = [n_held, held_total]
+                else:
+                    self.in_use -= held_total  # every row cancelled: the tensor goes now
                 done += len(b.reqs)
         return done
 
@@ -554,12 +639,23 @@ class Server:
         self._pool.shutdown(wait=True)
         self._jscratch.clear()
 
+    def step(self, now):
+        """One turn of the loop at server time ``now``: retire finished
+        batches, load kernels compiled in the background, admit and launch.
+        Returns (requests that reached a final status, batches launched)."""
+        done = self._retire(now) + self._settle()
+        self._load_compiled()
+        launched = self._admit_and_launch(now)
+        return done + self._settle(), launched
+
     def run(self, requests, realtime=True, clock=time.perf_counter, progress=None, progress_every=1000):
         """Serve ``requests`` (sorted by arrival).  realtime: a request
         becomes visible at its arrival time on the server clock; otherwise
         all are visible at once.  progress(finished, total, seconds) is
-        called every ``progress_every`` finished requests.  Returns the
-        results in request order."""
+        called every ``progress_every`` finished requests (it may call
+        ``cancel``).  Returns the results in request order; each has a
+        status: "ok", "cancelled" or "error" (a launch that raised fails its
+        own batch only; a GPU fault is not recoverable in the process)."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = clock()
         i, total, finished = 0, len(reqs), 0
@@ -568,13 +664,11 @@ class Server:
             while i < total and (not realtime or reqs[i].arrival <= now):
                 self.submit(reqs[i])
                 i += 1
-            done = self._retire(now)
+            done, launched = self.step(now)
             if progress is not None and done and (finished + done) // progress_every > finished // progress_every:
                 progress(finished + done, total, now)
-            finished += done
-            self._load_compiled()
-            launched = self._admit_and_launch(now)
-            if self.pending() and not self.inflight():
+            finished += done + self._settle()
+            if self.pending() and not self.inflight() and not done and not launched:
                 # nothing running and nothing fits: released states would
                 # have to make room
                 raise RuntimeError("HBM budget exhausted by resident states; release results")

@@ -151,3 +151,58 @@ def test_classes_merge_into_multi_plan_launches(sv):
     alone, _ = sv.run_one_at_a_time(reqs)
     for a, b in zip(mix, alone):
         _check_same(a, b)
+
+
+def test_cancellation_and_launch_failure_isolation(sv, monkeypatch):
+    """Requests cancelled in the queue and while running end "cancelled";
+    one QFIM request whose launch raises (the failure injected from the
+    host into its Jacobian call) ends "error" without touching the other
+    requests, which equal their alone results; the budget comes back to
+    zero once every kept state is released."""
+    import time
+    reqs = sv.make_workload(40, rate=0.0, seed=9, n_range=(6, 10), depth_range=(2, 5), qfim_frac=0.3)
+    alone, _ = sv.run_one_at_a_time(reqs)
+    bad = next(r for r in reqs if r.kind == sv.QFIM)
+    real = sv.E.jacobian
+
+    def jacobian(plan, params, init, **kw):
+        if np.array_equal(params.reshape(-1).cpu().numpy(), bad.params):
+            raise sv.E.EngineError("injected launch failure")
+        return real(plan, params, init, **kw)
+
+    monkeypatch.setattr(sv.E, "jacobian", jacobian)
+    srv = sv.Server(n_streams=2, max_batch=4)
+    for r in reqs:
+        srv.submit(r)
+    queued = [reqs[3].rid, reqs[17].rid]
+    assert bad.rid not in queued
+    for rid in queued:
+        assert srv.cancel(rid) == sv.CANCELLED
+    t0 = time.perf_counter()
+    finished, running = 0, None
+    while finished < len(reqs):
+        done, _ = srv.step(time.perf_counter() - t0)
+        finished += done
+        if running is None:
+            cand = [r.rid for r in reqs if srv.results[r.rid].launched is not None
+                    and srv.results[r.rid].status is None and r.rid != bad.rid]
+            if cand:
+                running = cand[0]
+                assert srv.cancel(running) == sv.CANCELLED
+    assert running is not None
+    for r, a in zip(reqs, alone):
+        res = srv.results[r.rid]
+        if r.rid in queued:
+            assert res.status == sv.CANCELLED and res.launched is None
+        elif r.rid == running:
+            assert res.status == sv.CANCELLED and res.state is None and res.Q is None and res.qfim is None
+        elif r.rid == bad.rid:
+            assert res.status == sv.ERROR and "injected" in res.error and res.qfim is None
+        else:
+            assert res.status == sv.OK
+            _check_same(res, a)
+    assert sv.result_from_json(sv.result_to_json(srv.results[bad.rid]))["status"] == sv.ERROR
+    for r in reqs:
+        srv.release(r.rid)
+    assert srv.in_use == 0
+    srv.close()

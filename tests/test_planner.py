@@ -83,7 +83,7 @@ def test_entry_points_reject_null_context_without_touching_the_gpu():
     import ctypes as C
     lib = L.lib()
     skip = {"pqs_abi_version", "pqs_last_error", "pqs_ctx_create", "pqs_ctx_destroy", "pqs_plan_destroy",
-            "pqs_plan_create", "pqs_batch_table_bytes", "pqs_jit_prepare"}
+            "pqs_plan_create", "pqs_batch_table_bytes", "pqs_jit_prepare", "pqs_range_pop"}
     checked = 0
     for name, (res, args) in L.SIGNATURES.items():
         if name in skip:
@@ -760,3 +760,22 @@ def test_exchange_transitions_compile_and_keep_layouts(pyqc, tmp_path, monkeypat
     assert sum(s.count(b"lswap<") for s in plan2.jit_sweep_sources()) == 0
     E._compile_in_children(srcs[:2], 2)
     plan.jit(load=False, workers=1)
+
+
+def test_profiler_ranges_without_and_with_roctx():
+    """pqs_range_push / pop: no-ops without PQS_ROCTX, a NULL name is
+    rejected; with PQS_ROCTX=1 (a fresh process) the library loads
+    the roctx library and nested ranges push and pop without a GPU."""
+    import os
+    import subprocess
+    import sys
+    lib = L.lib()
+    assert lib.pqs_range_push(b"outer") == 0 and lib.pqs_range_pop() == 0
+    assert lib.pqs_range_push(None) == L.PQS_ERR_ARG
+    code = ("import ctypes, sys; lib = ctypes.CDLL(sys.argv[1]); "
+            "rc = [lib.pqs_range_push(b'a'), lib.pqs_range_push(b'b'), lib.pqs_range_pop(), lib.pqs_range_pop()]; "
+            "assert rc == [0, 0, 0, 0], rc; print('ok')")
+    env = dict(os.environ, PQS_ROCTX="1")
+    out = subprocess.run([sys.executable, "-c", code, L.LIB_PATH], env=env, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0 and "ok" in out.stdout, out.stderr
