@@ -77,6 +77,17 @@ class PQC:
         self.n_layers += n
         self.set_gates()
 
+    def add_layers(self, layers, copy=True):
+        """Append several layers with one gate-list rebuild (``add_layer``
+        per layer rebuilds the list each time).  copy=False keeps the given
+        gate objects instead of deep copies: only for layers whose gate
+        objects are all distinct and held by nothing else (a template's
+        list repeats one layer object, so it needs the copies)."""
+        layers = list(layers)
+        self.layers.extend(deepcopy(l) for l in layers) if copy else self.layers.extend(layers)
+        self.n_layers += len(layers)
+        self.set_gates()
+
     def set_layer(self, layer, pos):
         """circuit.py:45-48."""
         self.layers[pos] = deepcopy(layer)

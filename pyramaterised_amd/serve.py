@@ -30,6 +30,7 @@ own state.
 """
 import base64
 import json
+import threading
 import time
 from collections import OrderedDict, deque
 from concurrent.futures import ThreadPoolExecutor
@@ -175,13 +176,15 @@ class _Circuit:
 
     def __init__(self, n, depth, dtype):
         self.qc = PQC(n, dtype=dtype, jit=False)
-        for layer in T.clifford_HE_layers(depth, n):
-            self.qc.add_layer(layer)
+        # one template call per layer: every layer has gate objects of its own
+        # (the template repeats one layer object, which add_layer deep-copies)
+        self.qc.add_layers([T.clifford_HE_layers(1, n)[0] for _ in range(depth)], copy=False)
         self.init = self.qc.init()
         self._plan = None
         self._gplan = None
         self._jit = {}
         self._pending = {}
+        self._lock = threading.Lock()  # plans are built by the preparing thread or on demand
 
     def specialise(self, which, pool, workers):
         if which in self._jit or which in self._pending:
@@ -212,15 +215,17 @@ class _Circuit:
 
     @property
     def plan(self):
-        if self._plan is None:
-            self._plan = self.qc.plan()
-        return self._plan
+        with self._lock:
+            if self._plan is None:
+                self._plan = self.qc.plan()
+            return self._plan
 
     @property
     def gplan(self):
-        if self._gplan is None:
-            self._gplan = self.qc.plan(with_generators=True)
-        return self._gplan
+        with self._lock:
+            if self._gplan is None:
+                self._gplan = self.qc.plan(with_generators=True)
+            return self._gplan
 
 
 class _Slot:
@@ -248,7 +253,7 @@ class Server:
 
     def __init__(self, hbm_budget=None, n_streams=2, max_batch=512, dtype="complex128", keep_states=True,
                  max_inflight_per_stream=2, specialise_qfim_from=14, specialise_state_from=18, compile_workers=8,
-                 large_bytes=1 << 30, merge_classes=True):
+                 large_bytes=1 << 30, merge_classes=True, prepare_workers=4):
         self.dev = E.device()
         if hbm_budget is None:
             _, total = torch.cuda.mem_get_info(self.dev)
@@ -286,6 +291,12 @@ class Server:
         self._order_dirty = False
         self._jscratch = {}       # slot -> reusable Jacobian buffer (stream-ordered reuse)
         self._cancel_running = set()  # rids cancelled while their batch runs
+        # a class's circuit and plan are built in the background from its
+        # first submitted request on (the planner's C++ part runs without
+        # the GIL); admission takes only classes whose plan is ready
+        self._prep_pool = ThreadPoolExecutor(max_workers=max(1, int(prepare_workers)))
+        self._prep = {}           # (n, depth, plan name) -> future
+        self._circuits_lock = threading.Lock()
         self._n_settled = 0       # requests cancelled in the queue or failed at launch, not yet counted
 
     # ---------------------------------------------------------------- sizing
@@ -303,10 +314,39 @@ class Server:
 
     def circuit(self, n, depth):
         key = (n, depth)
-        c = self.circuits.get(key)
-        if c is None:
-            c = self.circuits[key] = _Circuit(n, depth, self.dtype)
+        with self._circuits_lock:
+            c = self.circuits.get(key)
+            if c is None:
+                c = self.circuits[key] = _Circuit(n, depth, self.dtype)
         return c
+
+    @staticmethod
+    def _plan_name(kind):
+        return "gplan" if kind == QFIM else "plan"
+
+    def _prepare(self, n, depth, which):
+        getattr(self.circuit(n, depth), which)
+
+    def _ready(self, key, now):
+        """The class's plan is built.  A class whose preparation raised has
+        its queued requests failed (status "error")."""
+        n, depth, kind = key
+        f = self._prep.get((n, depth, self._plan_name(kind)))
+        if f is None or not f.done():
+            return False
+        err = f.exception()
+        if err is None:
+            return True
+        q = self.queues[key]
+        reqs = list(q)
+        q.clear()
+        self._n_pending -= len(reqs)
+        self._order_dirty = True
+        self._fail(reqs, 0, err, now)
+        return False
+
+    def _preparing(self):
+        return any(not f.done() for f in self._prep.values())
 
     # ---------------------------------------------------------------- queue
     def submit(self, req):
@@ -317,6 +357,9 @@ class Server:
         self.queues.setdefault((req.n, req.depth, req.kind), deque()).append(req)
         self._n_pending += 1
         self._order_dirty = True
+        pk = (req.n, req.depth, self._plan_name(req.kind))
+        if pk not in self._prep:
+            self._prep[pk] = self._prep_pool.submit(self._prepare, *pk)
 
     def pending(self):
         return self._n_pending
@@ -424,7 +467,7 @@ class Server:
             return 0
         take, need = [], 0
         for key in self._order:
-            if not self._mergeable(key):
+            if not self._mergeable(key) or not self.queues[key] or not self._ready(key, now):
                 continue
             q = self.queues[key]
             while q and len(take) < self.max_batch:
@@ -458,7 +501,7 @@ class Server:
             if self._mergeable(key):
                 continue
             q = self.queues[key]
-            if not q:
+            if not q or not self._ready(key, now):
                 continue
             slot = self._free_slot(self.is_large(q[0]))
             if slot is None:
@@ -636,6 +679,7 @@ class Server:
                 self._compiling.discard(key)
 
     def close(self):
+        self._prep_pool.shutdown(wait=True)
         self._pool.shutdown(wait=True)
         self._jscratch.clear()
 
@@ -668,9 +712,9 @@ class Server:
             if progress is not None and done and (finished + done) // progress_every > finished // progress_every:
                 progress(finished + done, total, now)
             finished += done + self._settle()
-            if self.pending() and not self.inflight() and not done and not launched:
-                # nothing running and nothing fits: released states would
-                # have to make room
+            if self.pending() and not self.inflight() and not done and not launched and not self._preparing():
+                # nothing running, nothing being prepared and nothing fits:
+                # released states would have to make room
                 raise RuntimeError("HBM budget exhausted by resident states; release results")
             if not done and not launched:
                 time.sleep(50e-6)  # idle: yield the host thread briefly

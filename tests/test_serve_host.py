@@ -82,3 +82,25 @@ def test_wire_format_round_trips_bit_exact():
     sres = SV.Result(SV.Request(8, 2, 1, SV.STATE_MW, rng.random(4)))
     sres.Q = 0.123456789012345678
     assert SV.result_from_json(SV.result_to_json(sres))["Q"] == sres.Q
+
+
+def test_server_circuits_equal_the_layer_by_layer_build():
+    """The server builds each (n, depth) class with one template call per
+    layer and no copies; its op list (with derivative ops) equals the
+    reference-style build (add_layer per layer, deep copies)."""
+    import pyramaterised_amd as pyqc
+    from pyramaterised_amd import serve as SV
+    from pyramaterised_amd import templates as T
+
+    def key(ops):
+        return [(o.kind, o.q0, o.q1, o.slot, o.slot2, o.tangent, o.pauli, o.scale, o.offset, tuple(o.m))
+                for o in ops]
+
+    for n, d in ((2, 1), (5, 3), (9, 7)):
+        c = SV._Circuit(n, d, 0)
+        ref = pyqc.PQC(n)
+        for layer in T.clifford_HE_layers(d, n):
+            ref.add_layer(layer)
+        assert c.qc.n_true_params == ref.n_true_params == 2 * n * d
+        assert key(c.qc.ops(True)) == key(ref.ops(True))
+        assert len({id(g) for g in c.qc.gates}) == len(c.qc.gates)
