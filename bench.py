@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12   # MI355X HBM3E bytes/s
 PEAK_FP64 = 78.6e12  # MI355X FP64 flop/s (vector and matrix rates are equal)
+PEAK_FP32 = 157.3e12  # MI355X FP32 flop/s
 
 
 def log(*a):
@@ -169,13 +170,19 @@ def leg_28q(pyqc, dtype, steps=3):
     tm.stop()
     ms = tm.ms() / steps
     b = sweep_bytes(plan, 1, amp)
+    flops = algorithmic_flops(qc)
     norm = float(torch.linalg.vector_norm(out).item())
     del out
     torch.cuda.empty_cache()
+    # both rooflines: the sweeps stream the state 11 times and carry 840
+    # rotations (FP64 vector work for complex128, FP32 for complex64)
+    peak_fl = PEAK_FP64 if dtype == "complex128" else PEAK_FP32
     return {"dtype": dtype, "ms": ms, "gates_per_s": 1245 / (ms * 1e-3), "n_sweeps": plan.n_sweeps,
             "tile_bits": plan.tile_bits, "sweep_launch_ms": ms / plan.n_sweeps,
             "jit": bool(getattr(plan, "jitted", False)),
-            "GBps": b / (ms * 1e-3) / 1e9, "hbm_frac": b / (ms * 1e-3) / PEAK_HBM, "norm": norm}
+            "GBps": b / (ms * 1e-3) / 1e9, "hbm_frac": b / (ms * 1e-3) / PEAK_HBM,
+            "TFLOPs": flops / (ms * 1e-3) / 1e12, "flop_frac": flops / (ms * 1e-3) / peak_fl,
+            "flop_peak": "fp64 78.6 TFLOP/s" if dtype == "complex128" else "fp32 157.3 TFLOP/s", "norm": norm}
 
 
 def leg_qfim12(pyqc):
