@@ -19,6 +19,9 @@ tensor whose axis k is qubit k.
 Pins: ``dense_unitary`` builds full operators by explicit bit arithmetic (and
 np.kron for one-qubit factors, the genFockOp construction of gates.py:39-42);
 tests check ``run`` against it at N <= 5 and against the reference's qg energy.
+``derivative_states_compat`` is checked against ``derivative_states`` (the same
+states where every gate before each fSim is parameterised, the looked-up
+gate's state or the plain state elsewhere) and against central differences.
 """
 import numpy as np
 
@@ -141,6 +144,38 @@ def derivative_states(gates, params, n, init=None):
                 if gj == gi:
                     psi = apply_deriv(psi, h.deriv(th, ph, which), n)
             out.append(psi)
+    return out
+
+
+def derivative_states_compat(gates, params, n, init=None):
+    """PQC.get_gradients as written (circuit.py:174-192), its index lookup
+    included: for the two-parameter gate at index ``count`` of the
+    parameterised gates, the second state is take_derivative(self.gates[count],
+    param=2) (circuit.py:188-189): the gate at position ``count`` of the
+    circuit with parameterised_derivative(2) -- a two-parameter gate's phi
+    derivative, a one-parameter gate's derivative (Gate.parameterised_derivative
+    falls back to derivative(), gates.py:96-97), a fixed gate's identity
+    (Gate.derivative, gates.py:93-94), which leaves the plain state.
+    take_derivative puts a copy with the same angles back in the circuit
+    (circuit.py:158,171), so the states do not depend on the copies."""
+    bound = bind(gates, params)
+    psi0 = zero_state(n) if init is None else np.asarray(init, dtype=np.complex128)
+
+    def state_with(gi, which):
+        psi = psi0.copy()
+        for gj, (h, ang) in enumerate(zip(gates, bound)):
+            th, ph = ang if ang is not None else (0.0, 0.0)
+            psi = apply_prims(psi, h.prims(th, ph), n)
+            if gj == gi and which > 0:
+                psi = apply_deriv(psi, h.deriv(th, ph, which), n)
+        return psi
+
+    out = []
+    for count, gi in enumerate([i for i, g in enumerate(gates) if g.n_params > 0]):
+        out.append(state_with(gi, 1))
+        if gates[gi].n_params == 2:
+            x = gates[count]
+            out.append(state_with(count, min(x.n_params, 2)))
     return out
 
 

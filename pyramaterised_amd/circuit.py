@@ -494,12 +494,40 @@ class PQC:
             return self._special_plan(t), 0
         return self.plan(with_generators=True), t
 
-    def get_gradients(self, method="forward"):
+    def get_gradients(self, method="forward", compat=False):
         """circuit.py:174-192: every derivative state.  ``forward`` spawns all
         tangents from one forward pass (pqs_jacobian); ``plain`` re-simulates
-        the circuit once per parameter, as the reference does."""
-        _, J = self.jacobian(method=method)
-        return [Ket(J[i], self.n_qubits) for i in range(J.shape[0])]
+        the circuit once per parameter, as the reference does.
+
+        compat=True reproduces the reference's two-parameter branch
+        (circuit.py:186-191): the second row of the two-parameter gate at
+        index ``count`` of the parameterised gates is taken of
+        ``self.gates[count]``, the gate at that position of the circuit,
+        with its second-parameter derivative (``parameterised_derivative(2)``,
+        which a one-parameter gate answers with its derivative and a fixed
+        gate with the identity).  Any parameterised gate there other than
+        the fSim itself was handled earlier in the loop, so the reference
+        never fails on it; the row is then that gate's derivative state (the
+        state itself for a fixed gate).  Without compat, the second row is
+        the fSim's own phi derivative."""
+        psi, J = self.jacobian(method=method)
+        rows = [J[i] for i in range(J.shape[0])]
+        if compat:
+            params = [g for g in self.gates if g.param_count > 0]
+            for count, g in enumerate(params):
+                if g.param_count != 2:
+                    continue
+                x = self.gates[count]
+                if x is g:
+                    continue
+                row = self._gate_slot[id(g)] + 1
+                if x.param_count == 2:
+                    rows[row] = J[self._gate_slot[id(x)] + 1]
+                elif x.param_count == 1:
+                    rows[row] = J[self._gate_slot[id(x)]]
+                else:
+                    rows[row] = psi.reshape(-1)
+        return [Ket(r, self.n_qubits) for r in rows]
 
     def jacobian(self, method="forward"):
         """(psi, J) with J[t] = d psi / d theta_t as tensors.  Rows of

@@ -275,3 +275,33 @@ def test_arbgate_dense_path(pyqc, n):
     v = ref / np.linalg.norm(ref)
     out = (g * pyqc.qobj.to_ket(v, n)).full().ravel()
     assert np.max(np.abs(out - OS.apply_prims(v, OC.ARBGATE(H1, n).prims(0.8), n))) < 1e-12
+
+
+def test_get_gradients_compat_index_lookup(pyqc):
+    """get_gradients(compat=True) reproduces the reference's two-parameter
+    branch (circuit.py:186-191: the second row of the fSim at parameterised
+    index count is taken of self.gates[count]) against the oracle's
+    restatement of that loop; compat=False gives the standard rows."""
+    n = 3
+    specs = [
+        ([pyqc.H(0, n), pyqc.R_y(0, n), pyqc.fSim([0, 1], n), pyqc.CNOT([1, 2], n), pyqc.fSim([1, 2], n)],
+         [OC.H(0, n), OC.R_y(0, n), OC.fSim((0, 1), n), OC.CNOT((1, 2), n), OC.fSim((1, 2), n)]),
+        ([pyqc.CNOT([0, 1], n), pyqc.fSim([0, 1], n), pyqc.R_x(2, n)],
+         [OC.CNOT((0, 1), n), OC.fSim((0, 1), n), OC.R_x(2, n)]),
+        ([pyqc.R_y(0, n), pyqc.fSim([0, 1], n), pyqc.H(1, n), pyqc.R_z(1, n), pyqc.fSim([1, 2], n)],
+         [OC.R_y(0, n), OC.fSim((0, 1), n), OC.H(1, n), OC.R_z(1, n), OC.fSim((1, 2), n)]),
+    ]
+    rng = np.random.default_rng(77)
+    for gates, og in specs:
+        qc = pyqc.PQC(n)
+        qc.add_layer(gates)
+        th = rng.random(qc.n_true_params) * 2 * np.pi
+        qc.set_params(list(th))
+        want = OS.derivative_states_compat(og, th, n)
+        got = qc.get_gradients(compat=True)
+        assert len(got) == len(want)
+        for a, b in zip(got, want):
+            assert np.max(np.abs(a.full().ravel() - b)) < 1e-12
+        std = OS.derivative_states(og, th, n)
+        for a, b in zip(qc.get_gradients(), std):
+            assert np.max(np.abs(a.full().ravel() - b)) < 1e-12

@@ -376,3 +376,53 @@ def test_arbgate_restatement_pins():
     h = 1e-6
     fd = (S.run(gates, th + [0, h, 0], n) - S.run(gates, th - [0, h, 0], n)) / (2 * h)
     assert np.max(np.abs(d[1] - 0.5 * fd)) < 1e-8
+
+
+def _quirk_circuits():
+    # (gates, n, expected map: row -> ("d", i) derivative_states row i, or "psi")
+    n = 3
+    a = [C.H(0, n), C.R_y(0, n), C.fSim((0, 1), n), C.CNOT((1, 2), n), C.fSim((1, 2), n)]
+    # parameterised: R_y (rows 0), fSim01 (1, 2), fSim12 (3, 4); fSim01 is
+    # parameterised gate 1 -> gates[1] = R_y; fSim12 is gate 2 -> gates[2] = fSim01
+    b = [C.CNOT((0, 1), n), C.fSim((0, 1), n), C.R_x(2, n)]
+    # fSim is parameterised gate 0 -> gates[0] = CNOT (fixed): the plain state
+    c = [C.R_y(0, n), C.fSim((0, 1), n), C.H(1, n), C.R_z(1, n), C.fSim((1, 2), n)]
+    # rows: R_y 0, fSim01 1-2, R_z 3, fSim12 4-5; fSim01 is parameterised
+    # gate 1 -> gates[1] = itself; fSim12 is gate 3 -> gates[3] = R_z
+    return [(a, n, {2: ("d", 0), 4: ("d", 2)}), (b, n, {1: "psi"}), (c, n, {5: ("d", 3)})]
+
+
+def test_compat_gradients_restate_the_index_lookup():
+    """derivative_states_compat against derivative_states: rows other than
+    the looked-up ones are equal; the looked-up row of a two-parameter gate
+    equals the derivative state of the gate at that circuit position (its
+    phi state for a two-parameter gate) or the plain state for a fixed
+    gate; with every gate before each fSim parameterised the two agree."""
+    rng = np.random.default_rng(31)
+    for gates, n, remap in _quirk_circuits():
+        th = rng.random(S.n_params(gates)) * 2 * np.pi
+        std = S.derivative_states(gates, th, n)
+        cmp = S.derivative_states_compat(gates, th, n)
+        psi = S.run(gates, th, n)
+        assert len(cmp) == len(std)
+        for r in range(len(std)):
+            want = std[r] if r not in remap else (psi if remap[r] == "psi" else std[remap[r][1]])
+            assert np.max(np.abs(cmp[r] - want)) < 1e-14, r
+    gates = [C.fSim((0, 1), 2), C.R_y(0, 2), C.fSim((0, 1), 2)]  # positions and indices coincide
+    th = rng.random(S.n_params(gates)) * 2 * np.pi
+    for x, y in zip(S.derivative_states_compat(gates, th, 2), S.derivative_states(gates, th, 2)):
+        assert np.max(np.abs(x - y)) < 1e-15
+    # a looked-up R_y row: central differences of the state (the fSim rows
+    # are the reference's derivative matrices, not derivatives; their
+    # blocks are pinned by test_fsim_derivative_matrices_against_central_differences)
+    gates, n, _ = _quirk_circuits()[0]
+    th = rng.random(S.n_params(gates)) * 2 * np.pi
+    h = 1e-6
+    std = S.derivative_states_compat(gates, th, n)
+    for row in (0, 2):  # R_y's own row, and fSim01's second row (looked up: R_y again)
+        p = 0
+        tp, tm = th.copy(), th.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (S.run(gates, tp, n) - S.run(gates, tm, n)) / (2 * h)
+        assert np.max(np.abs(fd - std[row])) < 1e-8
