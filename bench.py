@@ -109,14 +109,17 @@ def headline(pyqc, args):
     m = pyqc.measure.Measurements(qc)
     a_idx, b_idx = np.arange(0, S, 2), np.arange(1, S, 2)
     edges = np.linspace(0, 1, 76)
+    ov = torch.empty(n_pairs, dtype=torch.complex128, device="cuda")
 
     def step(timer=None):
+        # evolution of all 10,000 states (written to `states`) with the pair
+        # overlaps formed in the last sweep, then |.|^2 and the histogram
         if timer:
             timer.start()
-        E.run(plan, params, init, out=states)
+        E.run_pairs(plan, params, init, out=states, overlaps=ov)
         if timer:
             timer.stop()
-        F = E.abs2(E.inner_pairs(states, a_idx, b_idx))
+        F = E.abs2(ov)
         counts = E.histogram(F, edges).cpu().numpy()
         return counts, F
 
@@ -134,6 +137,10 @@ def headline(pyqc, args):
     evolve_ms = float(np.mean([t.ms() for t in timers]))
     gates = S * 940
     expr = m.expr(F, 2 ** 16, bins=75)
+    # the fused overlaps against a read-back of the written states
+    readback = E.inner_pairs(states, a_idx, b_idx)
+    pair_diff = float((readback - ov).abs().max().item())
+    fused = bool(getattr(plan, "jitted", False)) and any(b"pqs_jit_pair_" in x for x in plan.jit_sweep_sources())
     bytes_run = sweep_bytes(plan, S, 16)
     flops_run = algorithmic_flops(qc) * S
     launches = plan.n_sweeps
@@ -144,6 +151,7 @@ def headline(pyqc, args):
         "flops_per_launch": flops_run / launches, "achieved_flops": flops_run / (evolve_ms * 1e-3),
         "expr_kl": float(expr), "mean_F": float(F.mean().item()), "counts_nonzero": int((counts > 0).sum()),
         "S": S, "P": P, "jit": bool(getattr(plan, "jitted", False)), "plan_and_tune_s": jit_s,
+        "fused_pair_overlaps": fused, "pair_overlap_maxdiff_vs_readback": pair_diff,
     }
 
 
@@ -562,7 +570,7 @@ def main():
         "roofline": {"bound": "fp64-valu", "achieved": head["achieved_flops"] / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": head["achieved_flops"] / PEAK_FP64,
                      "traffic": traffic_per_launch(head),
-                     "kernel": "pqs_jit_sweep_* (run-time specialised sweeps, all launches of one evolution)",
+                     "kernel": "pqs_jit_sweep_* + pqs_jit_pair_* (run-time specialised sweeps, the last one forming the pair overlaps; all launches of one evolution)",
                      "launch_ms": head["sweep_launch_ms"], "flops_per_launch": head["flops_per_launch"],
                      "hbm_GBps": head["achieved_Bps"] / 1e9, "hbm_frac": head["achieved_Bps"] / PEAK_HBM},
         "cpu_baseline": cpu,

@@ -222,6 +222,21 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *plan, void *states_dev, int64_t state_
                   const int32_t *tangents_host, int32_t init, const void *src_dev,
                   const void *init_table_dev, void *stream);
 
+/* pqs_run_batch (plain evolution, no tangents) of an even batch whose
+ * instances pair up as (2i, 2i+1), with overlaps_dev[i] = <psi_2i|psi_2i+1>
+ * (complex128: 2 doubles per pair, batch / 2 pairs) besides the states.
+ * Replaces the fidelity of a pair of random states in
+ * Measurements._gen_f_samples (measure.py:123-137, psi.overlap(phi) at :135)
+ * when states are sampled in independent pairs.  A plan with run-time
+ * specialised sweeps forms the overlaps inside its last sweep (each block
+ * evolves the same tile of both instances of a pair and pairs their final
+ * amplitudes in registers), so the states are not read back; otherwise they
+ * are read back after the sweeps.  PQS_ERR_ARG for an odd batch.  Arguments
+ * otherwise as pqs_run_batch. */
+int pqs_run_batch_pairs(pqs_ctx *ctx, pqs_plan *plan, void *states_dev, int64_t state_stride, int64_t batch,
+                        const double *params_dev, int64_t param_stride, int32_t init, const void *src_dev,
+                        const void *init_table_dev, double *overlaps_dev, void *stream);
+
 /* Instances of several plans in one run: instance i evolves with plan
  * plans[inst_plan[i]] (fields as in pqs_run; init 2 takes the per-instance
  * product table init_tabs[i], init_tabs may be NULL otherwise).  The plans

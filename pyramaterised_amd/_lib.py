@@ -64,6 +64,7 @@ SIGNATURES = {
     "pqs_plan_tangent_parts": (C.c_int, [vp, i32, C.POINTER(i32)]),
     "pqs_run": (C.c_int, [vp, vp, C.POINTER(Instance), i64, vp, vp]),
     "pqs_run_batch": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, vp, i32, vp, vp, vp]),
+    "pqs_run_batch_pairs": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, i32, vp, vp, vp, vp]),
     "pqs_apply_dense": (C.c_int, [vp, i32, vp, i64, vp, i64, i64, vp, vp]),
     "pqs_apply_pauli": (C.c_int, [vp, i32, vp, i32, vp, vp, vp, i32, vp, vp]),
     "pqs_pauli_transition": (C.c_int, [vp, i32, vp, vp, i32, vp, vp, vp, i32, vp, vp]),

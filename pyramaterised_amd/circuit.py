@@ -448,6 +448,26 @@ class PQC:
             return self._run_dense(params.cpu().numpy() if torch.is_tensor(params) else params)
         return E.run(self.plan(batch=len(params)), params, self.init())
 
+    def run_batch_pairs(self, params):
+        """Evolve an even number of parameter vectors (S, P) and return
+        (states (S, 2^N), overlaps (S/2,) complex128), overlaps[i] =
+        <state_2i|state_2i+1>: the fidelity samples of independent pairs
+        (measure.py:123-137) without reading the states back where the plan's
+        sweeps are specialised.  The gate objects keep the last row's angles."""
+        params = np.asarray(params, dtype=np.float64) if not torch.is_tensor(params) else params
+        if len(params) % 2:
+            raise ValueError("run_batch_pairs needs an even number of parameter rows")
+        if len(params) == 0:
+            return (torch.empty((0, 2 ** self.n_qubits), dtype=E.CDTYPE[self.dtype], device=E.device()),
+                    torch.empty(0, dtype=torch.complex128, device=E.device()))
+        last = params[-1].tolist() if not torch.is_tensor(params) else params[-1].cpu().tolist()
+        self.set_params(last)
+        if self.has_dense():
+            st = self._run_dense(params.cpu().numpy() if torch.is_tensor(params) else params)
+            S = st.shape[0]
+            return st, E.inner_pairs(st, np.arange(0, S, 2), np.arange(1, S, 2))
+        return E.run_pairs(self.plan(batch=len(params)), params, self.init())
+
     def update_state(self, angles):
         """circuit.py:127-130."""
         self.state = self.run(angles)

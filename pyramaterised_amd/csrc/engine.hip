@@ -1074,6 +1074,27 @@ __global__ __launch_bounds__(RB) void k_inner_partial(const typename CT<R>::V *_
   if (threadIdx.x == 0) part[blockIdx.x] = double2{v[0], v[1]};
 }
 
+// <a_i|b_i> partials of a strided batch: a_i = base + 2 i stride, b_i = a_i + stride.
+template <typename R>
+__global__ __launch_bounds__(RB) void k_pair_partial(const typename CT<R>::V *__restrict__ base, int64_t stride,
+                                                     int64_t dim, int64_t n_chunks, double2 *__restrict__ part) {
+  __shared__ double sh[2 * RB];
+  const int64_t pair = blockIdx.x / n_chunks;
+  const int64_t chunk = blockIdx.x - pair * n_chunks;
+  const auto *a = base + 2 * pair * stride;
+  const auto *b = a + stride;
+  const int64_t lo = chunk * kInnerChunk;
+  const int64_t hi = lo + kInnerChunk < dim ? lo + kInnerChunk : dim;
+  double v[2] = {0, 0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += RB) {
+    const auto x = a[i], y = b[i];
+    v[0] += double(x.x) * y.x + double(x.y) * y.y;
+    v[1] += double(x.x) * y.y - double(x.y) * y.x;
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = double2{v[0], v[1]};
+}
+
 __global__ __launch_bounds__(RB) void k_sum_rows(const double2 *__restrict__ part, int64_t n_chunks,
                                                  double2 *__restrict__ out) {
   __shared__ double sh[2 * RB];
@@ -2260,9 +2281,14 @@ static int launch_add_rows(pqs_plan *p, const AccRow *d_acc, const void *const *
 //    instance runs out of place: its second buffer is a workspace row.  An
 //    instance whose input is its own output buffer, while the first sweep
 //    would write that buffer, first copies the input to its second buffer.
-// Workspace: [alt rows | part rows | instance table | AccRow table | sources].
+//  - pair_out (pqs_run_batch_pairs): also pair_out[i] = <psi_2i|psi_2i+1>
+//    (complex128) for i < h.size() / 2.  The last sweep's pair kernel forms
+//    the overlaps from its registers when the plan has one on this device;
+//    otherwise they are read back from the states after the sweeps.
+// Workspace: [alt rows | part rows | instance table | AccRow table | sources
+// | overlap partials].
 static int run_instances(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, const void *init_tab,
-                         hipStream_t st) {
+                         hipStream_t st, double *pair_out = nullptr) {
   const size_t sb = (size_t(1) << p->n) * amp_bytes(p->dtype);
   const size_t n_main = h.size();
   // extra instances for split generators: (owner, sub-tangent)
@@ -2292,7 +2318,16 @@ static int run_instances(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, con
   }
   const size_t acc_off = al(tab_off + n_all * sizeof(InstDev));
   const size_t src_off = al(acc_off + acc.size() * sizeof(AccRow));
-  const size_t total = src_off + srcs.size() * sizeof(void *);
+  const int S_sw = int(p->dev_sweeps.size());
+  const int64_t n_pairs = int64_t(n_main / 2);
+  const bool fused = pair_out && S_sw > 0 && parts.empty() && p->jit_pair_fn && p->jit_module &&
+                     p->dev_device == ctx->device && S_sw - 1 < int(p->jit_fn.size()) && p->jit_fn[S_sw - 1];
+  // partials: per pair and last-sweep tile (fused), else per pair and chunk
+  const int64_t n_chunks = !pair_out ? 0
+                           : fused   ? p->dev_sweeps[size_t(S_sw - 1)].n_tiles
+                                     : ((int64_t(1) << p->n) + kInnerChunk - 1) / kInnerChunk;
+  const size_t pov_off = al(src_off + srcs.size() * sizeof(void *));
+  const size_t total = pov_off + size_t(n_pairs * n_chunks) * sizeof(double2);
   int rc = ws_reserve(ctx, total, st);
   if (rc) return rc;
   char *ws = (char *)ctx->ws;
@@ -2331,7 +2366,7 @@ static int run_instances(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, con
     }
   }
   // one upload: instance table, AccRows, source pointers
-  std::vector<char> blob(total - tab_off, 0);
+  std::vector<char> blob(pov_off - tab_off, 0);
   std::memcpy(blob.data(), h.data(), h.size() * sizeof(InstDev));
   if (!acc.empty()) {
     std::memcpy(blob.data() + (acc_off - tab_off), acc.data(), acc.size() * sizeof(AccRow));
@@ -2340,12 +2375,45 @@ static int run_instances(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, con
   rc = upload(ctx, ws + tab_off, blob.data(), blob.size(), st);
   if (rc) return rc;
   const InstDev *d_tab = (const InstDev *)(ws + tab_off);
-  for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
+  double2 *pov = (double2 *)(ws + pov_off);
+  for (int s = 0; s < S_sw; ++s) {
+    if (fused && s == S_sw - 1) {
+      if (n_pairs == 0) break;
+      const DevSweep &sw = p->dev_sweeps[size_t(s)];
+      const int nthr = (1 << sw.k) >> pqs::kRegBits;
+      const int64_t grid = n_pairs * sw.n_tiles;
+      if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "grid too large: split the batch");
+      long long na = (long long)h.size();
+      void *args[] = {(void *)&d_tab, (void *)&na, (void *)&init_tab, (void *)&pov};
+      HIPCHECK(hipModuleLaunchKernel((hipFunction_t)p->jit_pair_fn, unsigned(grid), 1, 1, unsigned(nthr), 1, 1, 0,
+                                     st, args, nullptr));
+      break;
+    }
     rc = launch_sweep(ctx, p, s, d_tab, int64_t(h.size()), init_tab, st);
     if (rc) return rc;
   }
-  return launch_add_rows(p, (const AccRow *)(ws + acc_off), (const void *const *)(ws + src_off),
-                         int64_t(acc.size()), st);
+  rc = launch_add_rows(p, (const AccRow *)(ws + acc_off), (const void *const *)(ws + src_off),
+                       int64_t(acc.size()), st);
+  if (rc || !pair_out || n_pairs == 0) return rc;
+  if (!fused) {
+    // overlaps read back from the states: instance i's state is h[i].state
+    // (a strided batch: pair i at h[2i].state, stride h[1].state - h[0].state)
+    const int64_t dim = int64_t(1) << p->n;
+    const size_t ab = amp_bytes(p->dtype);
+    const int64_t stride = int64_t(((const char *)h[1].state - (const char *)h[0].state) / ptrdiff_t(ab));
+    const int64_t grid = n_pairs * n_chunks;
+    if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "too many pairs x chunks");
+    if (p->dtype == PQS_C128)
+      hipLaunchKernelGGL(k_pair_partial<double>, dim3(unsigned(grid)), dim3(RB), 0, st, (const double2 *)h[0].state,
+                         stride, dim, n_chunks, pov);
+    else
+      hipLaunchKernelGGL(k_pair_partial<float>, dim3(unsigned(grid)), dim3(RB), 0, st, (const float2 *)h[0].state,
+                         stride, dim, n_chunks, pov);
+    HIPCHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_sum_rows, dim3(unsigned(n_pairs)), dim3(RB), 0, st, pov, n_chunks, (double2 *)pair_out);
+  HIPCHECK(hipGetLastError());
+  return PQS_OK;
 }
 
 int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst, const void *init_tab,
@@ -2409,6 +2477,36 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride,
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
   return run_instances(ctx, p, h, init_tab, st);
+}
+
+int pqs_run_batch_pairs(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride, int64_t batch,
+                        const double *params, int64_t param_stride, int32_t init, const void *src,
+                        const void *init_tab, double *overlaps, void *stream) {
+  Range rg_("pqs_run_batch_pairs");
+  if (!ctx || !p || !states || !overlaps) return fail(PQS_ERR_ARG, "NULL ctx, plan, states or overlaps");
+  if (batch < 0 || (batch & 1)) return fail(PQS_ERR_ARG, "batch must be even and not negative");
+  if (batch == 0) return PQS_OK;
+  if (state_stride < (int64_t(1) << p->n)) return fail(PQS_ERR_ARG, "state stride below 2^N");
+  if (init < 0 || init > 2) return fail(PQS_ERR_ARG, "init must be 0, 1 or 2");
+  if (init == 2 && !init_tab) return fail(PQS_ERR_ARG, "product init needs an init table");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ab = amp_bytes(p->dtype);
+  std::vector<InstDev> h((size_t)batch);
+  for (int64_t i = 0; i < batch; ++i) {
+    InstDev &d = h[(size_t)i];
+    d.state = (char *)states + size_t(i) * size_t(state_stride) * ab;
+    d.src = src;
+    d.params = params ? params + i * param_stride : nullptr;
+    d.alt = nullptr;
+    d.tangent = -1;
+    d.init = init;
+    d.start_sweep = 0;
+    d.pingpong = 0;
+  }
+  int rc = plan_upload(ctx, p);
+  if (rc) return rc;
+  return run_instances(ctx, p, h, init_tab, st, overlaps);
 }
 
 int pqs_run_multi(pqs_ctx *ctx, pqs_plan *const *plans, int32_t n_plans, const pqs_instance *insts,
@@ -3411,6 +3509,8 @@ int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *p, int32_t load) {
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
   std::vector<void *> mods, fns(p->dev_sweeps.size(), nullptr);
+  void *pair_fn = nullptr;
+  const int last = int(p->dev_sweeps.size()) - 1;
   for (size_t i = 0; i < sw.size(); ++i) {
     hipModule_t mod;
     hipError_t e = hipModuleLoadData(&mod, code[i].data());
@@ -3421,6 +3521,13 @@ int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *p, int32_t load) {
       e = hipModuleGetFunction(&f, mod, name);
       mods.push_back((void *)mod);
       if (e == hipSuccess) fns[sw[i]] = (void *)f;
+      if (e == hipSuccess && sw[i] == last) {
+        // the pair form exists only where the emitter wrote one
+        std::snprintf(name, sizeof name, "pqs_jit_pair_%d", last);
+        hipFunction_t fp;
+        if (hipModuleGetFunction(&fp, mod, name) == hipSuccess) pair_fn = (void *)fp;
+        (void)hipGetLastError();
+      }
     }
     if (e != hipSuccess) {
       for (void *m : mods) (void)hipModuleUnload((hipModule_t)m);
@@ -3429,6 +3536,7 @@ int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *p, int32_t load) {
   }
   p->jit_modules = mods;
   p->jit_fn = fns;
+  p->jit_pair_fn = pair_fn;
   p->jit_module = mods.empty() ? nullptr : mods[0];
   return PQS_OK;
 }

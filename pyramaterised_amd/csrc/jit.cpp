@@ -257,19 +257,48 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       for (int a = 0; a < na; ++a) o << (a ? "," : "") << dbl(p.dev_ops[aops[a]].offset);
       o << "};\n";
     }
+    const bool persist = (p.flags & PQS_PLAN_JIT_PERSISTENT) != 0;
+    // The last sweep of a plan also comes as a pair kernel (pqs_jit_pair_<s>)
+    // when the plan has no tangent generators: block b evolves tile
+    // (b mod n_tiles) of instances 2 (b / n_tiles) and 2 (b / n_tiles) + 1 one
+    // after the other, keeps the first one's final amplitudes in registers
+    // and writes the tile's share of <psi_2i|psi_2i+1> to pov[b].  Both
+    // instances end in the same register / lane layout, so each thread pairs
+    // its own amplitudes.
+    const bool emit_pair = s == S_total - 1 && !persist && p.n_tangents == 0;
+  for (int pm = 0; pm < (emit_pair ? 2 : 1); ++pm) {
+    const bool pair = pm == 1;
     // Block b processes flat tile b (flat tile = instance * n_tiles + tile).
     // Persistent form (PQS_PLAN_JIT_PERSISTENT): block b processes tiles b,
     // b + G, b + 2G, ... and issues the loads of its next tile into
     // registers before the current tile's phases, so HBM traffic overlaps
     // compute when only one block fits per CU.
-    o << "extern \"C\" __global__ __launch_bounds__(" << nthr << ") void pqs_jit_sweep_" << s
-      << "(const InstDev *__restrict__ insts, long long n_active, const V *__restrict__ init_tab) {\n";
+    o << "extern \"C\" __global__ __launch_bounds__(" << nthr << ") void " << (pair ? "pqs_jit_pair_" : "pqs_jit_sweep_")
+      << s << "(const InstDev *__restrict__ insts, long long n_active, const V *__restrict__ init_tab"
+      << (pair ? ", double2 *__restrict__ pov" : "") << ") {\n";
     o << "  __shared__ V tile[" << S << "];\n";
     o << "  __shared__ double2 tf[" << (na > 0 ? na : 1) << "];\n";
     o << "  const int tid = threadIdx.x;\n";
-    o << "  const long long total = n_active << " << (p.n - k) << ";\n";
-    o << "  long long t = blockIdx.x;\n";
-    o << "  if (t >= total) return;\n";
+    if (pair) {
+      o << "  __shared__ double2 pred[" << std::max(1, nthr / 64) << "];\n";
+      o << "  const long long total = (n_active >> 1) << " << (p.n - k) << ";\n";
+      o << "  const long long pb = blockIdx.x;\n";
+      o << "  if (pb >= total) return;\n";
+      // keep: the first instance's final amplitudes; (ore, oim): this
+      // thread's share of the overlap.  Each pass takes its thread index
+      // through an empty asm, so that nothing derived from it (lane
+      // offsets, addresses) is hoisted out of the loop and held in
+      // registers across both passes.
+      o << "  V keep[NA];\n  double ore = 0.0, oim = 0.0;\n";
+      o << "#pragma unroll 1\n  for (int half = 0; half < 2; ++half) {\n";
+      o << "  int tidv = threadIdx.x;\n  asm volatile(\"\" : \"+v\"(tidv));\n  const int tid = tidv;\n";
+      o << "  const long long t = ((((pb >> " << (p.n - k) << ") << 1) | half) << " << (p.n - k) << ") | (pb & "
+        << (ds.n_tiles - 1) << "ll);\n";
+    } else {
+      o << "  const long long total = n_active << " << (p.n - k) << ";\n";
+      o << "  long long t = blockIdx.x;\n";
+      o << "  if (t >= total) return;\n";
+    }
     // address / source helpers for a flat tile index
     o << "  auto tile_base = [&](long long ft) -> u64 {\n";
     o << "    const u64 tile_id = u64(ft) & " << (ds.n_tiles - 1) << "ull;\n";
@@ -301,7 +330,6 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     o << "      if (" << s << " == in.start_sweep) { src = in.src ? (const V *)in.src : dst; init = in.src ? 0 : in.init; }\n";
     o << "      else src = dst;\n";
     o << "    }\n  };\n";
-    const bool persist = (p.flags & PQS_PLAN_JIT_PERSISTENT) != 0;
     // Direct load / store: the first (last) phase's registers come from (go
     // to) HBM without staging the tile in LDS; the planner made the tile
     // bits at read (write) positions 0 and 1 lane bits of that phase, so
@@ -339,6 +367,18 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
       << " - q)) & 1ull)]);\n";
     o << "        pre[m] = v;\n      }\n    }\n  };\n";
     o << "  V pre[NA];\n  fetch(t, pre);\n";
+    // the final amplitude w of register / slot m: kept (first instance) or
+    // paired with the kept one (second instance)
+    auto capture = [&](const std::string &m, const std::string &w) {
+      if (!pair) return;
+      o << "    if (half == 0) keep[" << m << "] = " << w << ";\n"
+        << "    else {\n"
+        << "      ore += double(keep[" << m << "].x) * double(" << w << ".x) + double(keep[" << m << "].y) * double(" << w
+        << ".y);\n"
+        << "      oim += double(keep[" << m << "].x) * double(" << w << ".y) - double(keep[" << m << "].y) * double(" << w
+        << ".x);\n"
+        << "    }\n";
+    };
     if (persist) o << "  for (; t < total; t += gridDim.x) {\n";
     else o << "  {\n";
     o << "  const InstDev in = insts[t >> " << (p.n - k) << "];\n";
@@ -586,6 +626,7 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
             if (v & (1 << b)) cm |= 1ull << wp[P.rb[b]];
           o << "    dst[gt ^ " << cm << "ull] = a[" << v << "];\n";
         }
+        for (int v = 0; v < NA_(); ++v) capture(std::to_string(v), "a[" + std::to_string(v) + "]");
         o << "  }\n";
         continue;
       }
@@ -611,7 +652,11 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     if (dstore) {
       // stored by the last phase
     } else if (!ds.rolling) {
-      for (int m = 0; m < NA_(); ++m) o << "  dst[dl | " << hp(m) << "ull] = tile[swz(tid + " << m * nthr << ")];\n";
+      for (int m = 0; m < NA_(); ++m) {
+        o << "  {\n    const V w = tile[swz(tid + " << m * nthr << ")];\n    dst[dl | " << hp(m) << "ull] = w;\n";
+        capture(std::to_string(m), "w");
+        o << "  }\n";
+      }
     } else {
       // write layout: store index bit i is tile bit sperm[i], at write
       // position wpos[sperm[i]] (ascending), complement bit j at cwpos[j]
@@ -630,11 +675,27 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
             cm |= 1ull << ds.wpos[ds.sperm[kt + b]];
             jm |= 1 << ds.sperm[kt + b];
           }
-        o << "    dst[wdl | " << cm << "ull] = tile[swz(jdl | " << jm << ")];\n";
+        o << "    {\n    const V w = tile[swz(jdl | " << jm << ")];\n    dst[wdl | " << cm << "ull] = w;\n";
+        capture(std::to_string(m), "w");
+        o << "    }\n";
       }
       o << "  }\n";
     }
-    o << "  }\n}\n";
+    if (!pair) {
+      o << "  }\n}\n";
+      continue;
+    }
+    // close the tile scope and the loop over the pair; sum the threads'
+    // shares in a fixed order (wave, then waves)
+    o << "  }\n  }\n";
+    o << "  for (int off = 32; off > 0; off >>= 1) {\n"
+         "    ore += __shfl_xor(ore, off);\n    oim += __shfl_xor(oim, off);\n  }\n";
+    o << "  if ((tid & 63) == 0) pred[tid >> 6] = double2{ore, oim};\n  __syncthreads();\n";
+    o << "  if (tid == 0) {\n    double2 sum = pred[0];\n";
+    if (nthr / 64 > 1)
+      o << "    for (int w = 1; w < " << nthr / 64 << "; ++w) { sum.x += pred[w].x; sum.y += pred[w].y; }\n";
+    o << "    pov[pb] = sum;\n  }\n}\n";
+  }
   }
 }
 

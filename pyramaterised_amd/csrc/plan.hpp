@@ -144,6 +144,7 @@ struct pqs_plan {
   void *jit_module = nullptr;        // non-null once loaded
   std::vector<void *> jit_modules;   // one module per specialised sweep
   std::vector<void *> jit_fn;        // per sweep; null: interpreted
+  void *jit_pair_fn = nullptr;       // pair kernel of the last sweep (pqs_run_batch_pairs)
 };
 
 namespace pqs {

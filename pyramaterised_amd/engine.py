@@ -448,6 +448,37 @@ def run(plan, params, init, tangents=None, out=None):
     return out
 
 
+def run_pairs(plan, params, init, out=None, overlaps=None):
+    """Evolve an even batch (params (B, P)) and return (states (B, 2^N),
+    overlaps (B/2,) complex128) with overlaps[i] = <state_2i|state_2i+1>
+    (pqs_run_batch_pairs: a plan with specialised sweeps forms the overlaps
+    in its last sweep, without reading the states back)."""
+    ctx = context()
+    dev = device()
+    p = _params_tensor(params, dev, rows=True)
+    if p.numel() == 0:
+        p = torch.zeros((max(1, p.shape[0]) if p.dim() == 2 else 1, 1), dtype=torch.float64, device=dev)
+    B = p.shape[0]
+    if B % 2:
+        raise ValueError("run_pairs needs an even batch")
+    dim = 1 << plan.n
+    cdt = CDTYPE[plan.dtype]
+    if out is None:
+        out = torch.empty((B, dim), dtype=cdt, device=dev)
+    elif out.shape != (B, dim) or out.dtype != cdt or not out.is_contiguous():
+        raise ValueError("out has the wrong shape, dtype or layout")
+    if overlaps is None:
+        overlaps = torch.empty(B // 2, dtype=torch.complex128, device=dev)
+    elif overlaps.shape != (B // 2,) or overlaps.dtype != torch.complex128 or not overlaps.is_contiguous():
+        raise ValueError("overlaps has the wrong shape, dtype or layout")
+    code, src, tab = init.device_args(plan.dtype, dev)
+    pstride = p.stride(0) if p.shape[0] > 1 else 0
+    check(L.lib().pqs_run_batch_pairs(ctx.handle, plan.handle, ptr(out), dim, B, ptr(p), pstride, code, ptr(src),
+                                      ptr(tab), ptr(overlaps), stream_ptr()))
+    _keepalive(src, tab, p)
+    return out, overlaps
+
+
 def run_multi(jobs):
     """Evolve batches of several plans in one multi-plan run (pqs_run_multi):
     jobs = [(plan, params (B, P) or (P,), Init)] -> list of (B, 2^N_plan)

@@ -162,8 +162,12 @@ class Measurements:
         Returns (expr, F samples tensor)."""
         if params is None:
             params = self.QC.random_params(2 * n_pairs)
-        states = self.sample_states(2 * n_pairs, params)
-        F = E.abs2(E.inner_pairs(states, np.arange(0, 2 * n_pairs, 2), np.arange(1, 2 * n_pairs, 2)))
+        if hasattr(self.QC, "run_batch_pairs"):
+            _, ov = self.QC.run_batch_pairs(params)
+            F = E.abs2(ov)
+        else:
+            states = self.sample_states(2 * n_pairs, params)
+            F = E.abs2(E.inner_pairs(states, np.arange(0, 2 * n_pairs, 2), np.arange(1, 2 * n_pairs, 2)))
         return self.expr(F, 2 ** self.QC.n_qubits, bins=bins), F
 
     def find_eff_H(self, circuit_f_samples, n):

@@ -475,9 +475,10 @@ def test_window_search_never_adds_sweeps(pyqc):
 
 
 def _phase_counts(plan):
-    """Register phases per sweep, counted in the specialised kernel sources
-    (each register phase declares its pending-flip flags once)."""
-    return [src.decode().count("int xf0 = 0") for src in plan.jit_sweep_sources()]
+    """Register phases per sweep, counted in the specialised sweep kernel's
+    source (each register phase declares its pending-flip flags once; the
+    last sweep's pair kernel, which repeats them, follows it)."""
+    return [src.decode().split("void pqs_jit_pair_")[0].count("int xf0 = 0") for src in plan.jit_sweep_sources()]
 
 
 def test_phase_search_never_adds_phases(pyqc):
