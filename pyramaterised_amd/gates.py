@@ -129,9 +129,19 @@ class LocalOperator:
         if not self.ops:
             return ket.copy()
         n = ket.n
-        plan = E.Plan(n, self.ops, dtype=ket.t.dtype)
-        has_gen = any(o.tangent >= 0 for o in self.ops)
-        params = np.zeros(max(1, _n_slots(self.ops)))
+        ops, params = self.ops, None
+        if _n_slots(ops) == 0:
+            ops, params = _angles_as_params(ops)
+        key = (n, str(ket.t.dtype), bytes((L.Op * len(ops))(*ops)))
+        plan = _APPLY_PLANS.pop(key, None)
+        if plan is None:
+            plan = E.Plan(n, ops, dtype=ket.t.dtype)
+            while len(_APPLY_PLANS) >= _APPLY_PLANS_MAX:
+                _APPLY_PLANS.pop(next(iter(_APPLY_PLANS)))
+        _APPLY_PLANS[key] = plan  # most recently used last
+        has_gen = any(o.tangent >= 0 for o in ops)
+        if params is None:
+            params = np.zeros(max(1, _n_slots(ops)))
         out = E.run(plan, params, E.Init.from_vector(ket.t), tangents=[0] if has_gen else None)
         return Ket(out[0], n)
 
@@ -147,6 +157,29 @@ class LocalOperator:
             v[i] = 1
             cols.append(self.apply_to(Ket(v, self.q_N)).full().ravel())
         return np.array(cols).T
+
+
+# Plans of ``op * ket`` products by op structure (n, dtype, ops with their
+# constant rotation angles taken out), least recently used first: a gate
+# applied again at other angles (``for g in gates: state = g * state``, the
+# columns of ``full()``) reuses its plan.
+_APPLY_PLANS = {}
+_APPLY_PLANS_MAX = 256
+_ROT_KINDS = (L.OP_RX, L.OP_RY, L.OP_RZ, L.OP_RZZ, L.OP_RXX, L.OP_RYY)
+
+
+def _angles_as_params(ops):
+    """Ops without parameter slots -> (the same ops with every rotation's
+    constant angle as parameter k of a params vector, params)."""
+    out, params = [], []
+    for o in ops:
+        c = L.Op.from_buffer_copy(o)
+        if c.kind in _ROT_KINDS and c.slot < 0:
+            c.slot, c.scale = len(params), 1.0
+            params.append(c.offset)
+            c.offset = 0.0
+        out.append(c)
+    return out, np.asarray(params if params else [0.0], dtype=np.float64)
 
 
 def _n_slots(ops):

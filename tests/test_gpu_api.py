@@ -91,19 +91,32 @@ def test_gate_times_ket_and_derivative(pyqc):
         (pyqc.fSim([1, 2], n), OC.fSim((1, 2), n)),
         (pyqc.fixed_fSim([3, 0], n), OC.fixed_fSim((3, 0), n)),
     ]
+    from pyramaterised_amd import gates as G
     for g, og in cases:
-        th, ph = 1.234, 0.567
-        if g.is_param:
-            g.set_theta(th)
-            if g.param_count == 2:
-                g.set_phi(ph)
-        out = (g * ket).full().ravel()
-        want = OS.apply_prims(v, og.prims(th, ph), n)
-        assert np.max(np.abs(out - want)) < 1e-13, og.name
-        if g.is_param:
-            d = (g.derivative() * (g * ket)).full().ravel()
-            dwant = OS.apply_deriv(want, og.deriv(th, ph, 1), n)
-            assert np.max(np.abs(d - dwant)) < 1e-13, og.name
+        for th, ph in ((1.234, 0.567), (-2.5, 3.0)):
+            if g.is_param:
+                g.set_theta(th)
+                if g.param_count == 2:
+                    g.set_phi(ph)
+            n_plans = len(G._APPLY_PLANS)
+            out = (g * ket).full().ravel()
+            want = OS.apply_prims(v, og.prims(th, ph), n)
+            assert np.max(np.abs(out - want)) < 1e-13, og.name
+            if th < 0 and "fSim" not in og.name:
+                # the same structure at other angles: the cached plan (an
+                # fSim's angles live in its dense 4x4, so it is new)
+                assert len(G._APPLY_PLANS) == n_plans, og.name
+            if g.is_param:
+                d = (g.derivative() * (g * ket)).full().ravel()
+                dwant = OS.apply_deriv(want, og.deriv(th, ph, 1), n)
+                assert np.max(np.abs(d - dwant)) < 1e-13, og.name
+    # operation.full(): the dense operator, column by column through the
+    # cached plan
+    g = pyqc.R_y(1, 3)
+    g.set_theta(0.7)
+    U = np.asarray(g.operation.full())
+    cols = [OS.apply_prims(np.eye(8, dtype=np.complex128)[:, j], OC.R_y(1, 3).prims(0.7, 0.0), 3) for j in range(8)]
+    assert np.max(np.abs(U - np.stack(cols, axis=1))) < 1e-14
     # operator algebra: (g1 * g2) * ket applies g2 first, as QuTiP's product
     a, b = pyqc.R_x(0, n), pyqc.CNOT([0, 1], n)
     a.set_theta(0.3)

@@ -206,3 +206,35 @@ def test_cancellation_and_launch_failure_isolation(sv, monkeypatch):
         srv.release(r.rid)
     assert srv.in_use == 0
     srv.close()
+
+
+def test_wire_format_round_trip_of_served_results(sv):
+    """Requests read from JSON lines are served on the GPU; every result
+    written as a JSON line (state included) and read back equals the served
+    one bit for bit, and matches the oracle."""
+    reqs = sv.make_workload(12, rate=0.0, seed=9, n_range=(6, 10), depth_range=(2, 5), qfim_frac=0.4)
+    lines = [sv.request_to_json(r) for r in reqs]
+    back = [sv.request_from_json(x) for x in lines]
+    srv = sv.Server(n_streams=2, max_batch=4)
+    served = srv.run(back, realtime=False)
+    kinds = set()
+    for r, res in zip(reqs, served):
+        d = sv.result_from_json(sv.result_to_json(res, with_state=True))
+        assert d["rid"] == r.rid and d["status"] == sv.OK and d["kind"] == r.kind
+        og = OC.clifford_he(r.depth, r.n)
+        ref = OS.run(og, r.params, r.n)
+        kinds.add(r.kind)
+        if r.kind == "state_mw":
+            st = res.state.detach().cpu().numpy().ravel()
+            assert np.array_equal(d["state"].ravel(), st)          # bit-exact through the wire
+            assert d["Q"] == res.Q
+            assert infidelity(d["state"].ravel(), ref) < 1e-12
+            assert abs(d["Q"] - OM.single_Q(ref, r.n)) < 1e-12
+        else:
+            assert np.array_equal(d["qfim"], np.asarray(res.qfim, dtype=np.float64))
+            F = OM.qfim(OS.derivative_states(og, r.params, r.n), ref)
+            assert rel_fro(d["qfim"], F) < 1e-9
+    assert kinds == {"state_mw", "qfim"}
+    for r in reqs:
+        srv.release(r.rid)
+    srv.close()
