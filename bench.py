@@ -102,7 +102,7 @@ def headline(pyqc, args):
     rng = np.random.default_rng(2024)
     params = torch.as_tensor(rng.random((S, P)) * 2 * np.pi, device="cuda")
     t0 = time.perf_counter()
-    plan = qc.plan(batch=S)
+    plan = qc.plan(batch=S, pairs=True)
     jit_s = time.perf_counter() - t0
     init = qc.init()
     states = torch.empty((S, 2 ** qc.n_qubits), dtype=torch.complex128, device="cuda")

@@ -225,7 +225,10 @@ class PQC:
     # even for small batches (repeated QFIMs, training loops)
     JIT_AFTER_USES = 3
 
-    def plan(self, with_generators=False, dtype=None, batch=1):
+    def plan(self, with_generators=False, dtype=None, batch=1, pairs=False):
+        """The plan of this circuit (cached per generator mode and dtype);
+        ``batch`` and ``pairs`` (the caller runs pair batches,
+        ``run_batch_pairs``) describe the use the tile-shape tuner times."""
         dt = self.dtype if dtype is None else E.dtype_code(dtype)
         key = (with_generators, dt)
         p = self._plans.get(key)
@@ -240,7 +243,7 @@ class PQC:
         ops = self.ops(with_generators)
         if want and self._want_tune(with_generators):
             # specialised kernels: time the candidate tile shapes once
-            p = E.tuned_plan(self.n_qubits, ops, dt, batch, self.init())
+            p = E.tuned_plan(self.n_qubits, ops, dt, batch, self.init(), pairs=pairs and not with_generators)
         else:
             if p is None:
                 p = E.Plan(self.n_qubits, ops, dtype=dt, **self.plan_options(with_generators, dt))
@@ -466,7 +469,7 @@ class PQC:
             st = self._run_dense(params.cpu().numpy() if torch.is_tensor(params) else params)
             S = st.shape[0]
             return st, E.inner_pairs(st, np.arange(0, S, 2), np.arange(1, S, 2))
-        return E.run_pairs(self.plan(batch=len(params)), params, self.init())
+        return E.run_pairs(self.plan(batch=len(params), pairs=True), params, self.init())
 
     def update_state(self, angles):
         """circuit.py:127-130."""

@@ -184,18 +184,22 @@ def _n_params(ops):
     return 1 + max([-1] + [max(o.slot, o.slot2) for o in ops])
 
 
-def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
+def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None, pairs=False):
     """Build the plan for each tile shape of ``tile_candidates``, specialise
     all of them, time evolutions of each on this GPU (random angles, up to
-    2^26 amplitudes of the batch; the best of three interleaved rounds) and
-    return the first candidate within TUNE_TOLERANCE of the fastest.  The
-    choice is
-    cached in memory and in the code-object cache directory, keyed by the op
-    list, the dtype and the timed batch size."""
+    2^27 amplitudes of the batch; the best of three interleaved rounds) and
+    return the first candidate within TUNE_TOLERANCE of the fastest.  With
+    ``pairs`` the timed call is the pair run (``run_pairs``: the last sweep
+    as the pair kernel), which the caller will make.  The choice is cached
+    in memory and in the code-object cache directory, keyed by the op list,
+    the dtype, the timed batch size and the call."""
     cands = tile_candidates(n_qubits, dtype_c)
     arr = (L.Op * max(1, len(ops)))(*ops)
-    B = 1 if n_qubits >= 26 else max(1, min(int(batch), (1 << 26) >> n_qubits))
-    key = hashlib.sha1(bytes(arr) + repr((n_qubits, dtype_c, B, cands)).encode() +
+    B = 1 if n_qubits >= 26 else max(1, min(int(batch), (1 << 27) >> n_qubits))
+    pairs = bool(pairs) and B >= 2
+    if pairs:
+        B -= B % 2
+    key = hashlib.sha1(bytes(arr) + repr((n_qubits, dtype_c, B, cands, pairs)).encode() +
                        L.library_fingerprint().encode()).hexdigest()[:24]
     choice = _tune_memo.get(key)
     path = os.path.join(jit_cache_dir(), f"tune_{key}.json")
@@ -219,28 +223,36 @@ def tuned_plan(n_qubits, ops, dtype_c, batch, init, workers=None):
     g = torch.Generator(device="cpu").manual_seed(1)
     th = (torch.rand((B, max(1, _n_params(ops))), generator=g, dtype=torch.float64) * 2 * np.pi).to(dev)
     out = torch.empty((B, 1 << n_qubits), dtype=CDTYPE[dtype_c], device=dev)
+    ov = torch.empty(max(1, B // 2), dtype=torch.complex128, device=dev)
+
+    def call(p):
+        if pairs:
+            run_pairs(p, th, init, out=out, overlaps=ov)
+        else:
+            run(p, th, init, out=out)
+
     # three interleaved rounds, the fastest of each candidate: a single pass
     # in candidate order favours the later ones (the first timings of a
     # process run while the clocks ramp up)
     times = [float("inf")] * len(plans)
     for _ in range(3):
         for i, p in enumerate(plans):
-            run(p, th, init, out=out)
+            call(p)
             s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             s.record()
             for _ in range(2):
-                run(p, th, init, out=out)
+                call(p)
             e.record()
             e.synchronize()
             times[i] = min(times[i], s.elapsed_time(e))
-    del out
+    del out, ov
     best = pick_candidate(times)
     _tune_memo[key] = cands[best]
     try:
         os.makedirs(jit_cache_dir(), exist_ok=True)
         with open(path, "w") as f:
             json.dump({"shape": list(cands[best]), "candidates": [list(c) for c in cands],
-                       "ms_per_run": [t / 2 for t in times], "batch": B}, f)
+                       "ms_per_run": [t / 2 for t in times], "batch": B, "pairs": pairs}, f)
     except OSError:
         pass
     return plans[best]
