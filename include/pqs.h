@@ -130,19 +130,12 @@ enum {
                                    seeded with 4 consecutive tile qubits and
                                    keep the one after which the greedy pass
                                    needs the fewest phases) */
-  PQS_PLAN_NO_EXCHANGE = 256,   /* specialised kernels: every register-phase
+  PQS_PLAN_NO_EXCHANGE = 256    /* specialised kernels: every register-phase
                                    transition goes through LDS (default: a
                                    phase whose register bits differ from the
                                    previous phase's by one or two tile bits
                                    held on lane bits 5 / 4 there trades them
                                    with cross-lane swaps, no LDS round trip) */
-  PQS_PLAN_NO_PRODUCT_START = 512 /* specialised kernels: the first sweep
-                                   applies every gate to the generated start
-                                   state (default: runs that start from
-                                   |0...0> or a product table fold each
-                                   qubit's leading one-qubit gates into its
-                                   2-vector and generate the tile as a
-                                   product) */
 };
 
 /* One instance of a batched run. */

@@ -20,7 +20,6 @@ PQS_PLAN_NO_DIRECT = 32
 PQS_PLAN_GREEDY_SWEEPS = 64
 PQS_PLAN_GREEDY_PHASES = 128
 PQS_PLAN_NO_EXCHANGE = 256
-PQS_PLAN_NO_PRODUCT_START = 512
 
 (OP_MAT1, OP_RX, OP_RY, OP_RZ, OP_MAT2, OP_CX, OP_CZ, OP_RXX, OP_RYY, OP_RZZ,
  OP_FSIM, OP_GEN, OP_GEN_FSIM) = range(13)

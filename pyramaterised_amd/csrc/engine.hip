@@ -2376,23 +2376,7 @@ static int run_instances(pqs_ctx *ctx, pqs_plan *p, std::vector<InstDev> &h, con
   if (rc) return rc;
   const InstDev *d_tab = (const InstDev *)(ws + tab_off);
   double2 *pov = (double2 *)(ws + pov_off);
-  // sweep 0 from the product-start kernel when every instance starts there
-  // from |0...0> or a product table (no vector input, no split parts)
-  bool prod = S_sw > 1 && parts.empty() && p->jit_prod_fn && p->jit_module && p->dev_device == ctx->device &&
-              !p->jit_fn.empty() && p->jit_fn[0];
-  for (size_t i = 0; prod && i < h.size(); ++i) prod = h[i].init != 0 && h[i].start_sweep == 0;
   for (int s = 0; s < S_sw; ++s) {
-    if (prod && s == 0) {
-      const DevSweep &sw = p->dev_sweeps[0];
-      const int nthr = (1 << sw.k) >> pqs::kRegBits;
-      const int64_t grid = sw.n_tiles * int64_t(h.size());
-      if (grid > 0x7fffffff) return fail(PQS_ERR_UNSUPPORTED, "grid too large: split the batch");
-      long long na = (long long)h.size();
-      void *args[] = {(void *)&d_tab, (void *)&na, (void *)&init_tab};
-      HIPCHECK(hipModuleLaunchKernel((hipFunction_t)p->jit_prod_fn, unsigned(grid), 1, 1, unsigned(nthr), 1, 1, 0,
-                                     st, args, nullptr));
-      continue;
-    }
     if (fused && s == S_sw - 1) {
       if (n_pairs == 0) break;
       const DevSweep &sw = p->dev_sweeps[size_t(s)];
@@ -3525,7 +3509,7 @@ int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *p, int32_t load) {
   int rc = plan_upload(ctx, p);
   if (rc) return rc;
   std::vector<void *> mods, fns(p->dev_sweeps.size(), nullptr);
-  void *pair_fn = nullptr, *prod_fn = nullptr;
+  void *pair_fn = nullptr;
   const int last = int(p->dev_sweeps.size()) - 1;
   for (size_t i = 0; i < sw.size(); ++i) {
     hipModule_t mod;
@@ -3544,11 +3528,6 @@ int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *p, int32_t load) {
         if (hipModuleGetFunction(&fp, mod, name) == hipSuccess) pair_fn = (void *)fp;
         (void)hipGetLastError();
       }
-      if (e == hipSuccess && sw[i] == 0) {
-        hipFunction_t fp;
-        if (hipModuleGetFunction(&fp, mod, "pqs_jit_prod_0") == hipSuccess) prod_fn = (void *)fp;
-        (void)hipGetLastError();
-      }
     }
     if (e != hipSuccess) {
       for (void *m : mods) (void)hipModuleUnload((hipModule_t)m);
@@ -3558,7 +3537,6 @@ int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *p, int32_t load) {
   p->jit_modules = mods;
   p->jit_fn = fns;
   p->jit_pair_fn = pair_fn;
-  p->jit_prod_fn = prod_fn;
   p->jit_module = mods.empty() ? nullptr : mods[0];
   return PQS_OK;
 }

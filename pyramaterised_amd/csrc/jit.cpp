@@ -266,66 +266,18 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     // instances end in the same register / lane layout, so each thread pairs
     // its own amplitudes.
     const bool emit_pair = s == S_total - 1 && !persist && p.n_tangents == 0;
-    // The first sweep of a plan without tangents also comes as a product-start
-    // kernel (pqs_jit_prod_0) for runs whose instances all start from
-    // |0...0> or a product table: every qubit's leading one-qubit gates (those
-    // before its first two-qubit gate) act on a product state, so they are
-    // applied exactly to that qubit's 2-vector (one thread per qubit, from
-    // the instance's angles), the tile is generated as the product of the
-    // 2-vectors, and the sweep runs the remaining ops.  folded: ops applied
-    // to the 2-vectors.  A gate moves to the front only past gates it
-    // commutes with: per qubit, state 0 = nothing before it yet, 1 = only
-    // gates diagonal on it (CNOT control, CZ, R_zz), which a diagonal
-    // one-qubit gate (R_z, diagonal matrix) passes, 2 = anything else.
-    std::vector<char> folded(size_t(ds.op_end - ds.op_begin), 0);
-    bool emit_prod = s == 0 && S_total >= 2 && !persist && p.n_tangents == 0 && !(p.flags & PQS_PLAN_NO_PRODUCT_START);
-    if (emit_prod) {
-      std::vector<int> st(64, 0);
-      int nf = 0;
-      for (int i = ds.op_begin; i < ds.op_end; ++i) {
-        const DevOp &d = p.dev_ops[i];
-        const bool one = (d.kind == PQS_OP_RX || d.kind == PQS_OP_RY || d.kind == PQS_OP_RZ ||
-                          d.kind == PQS_OP_MAT1) && d.g1 < 0 && d.g0 >= 0;
-        const bool diag1 = d.kind == PQS_OP_RZ ||
-                           (d.kind == PQS_OP_MAT1 && d.m[2] == 0.0 && d.m[3] == 0.0 && d.m[4] == 0.0 && d.m[5] == 0.0);
-        if (one) {
-          int &q = st[size_t(d.g0)];
-          if (q == 0 || (q == 1 && diag1)) {
-            folded[size_t(i - ds.op_begin)] = 1;
-            ++nf;
-          } else {
-            q = 2;
-          }
-          continue;
-        }
-        // two-qubit (or tangent) ops: diagonal on CNOT's control, CZ and
-        // R_zz's qubits; a tangent generator blocks its qubit
-        const bool dg0 = d.kind == PQS_OP_CX || d.kind == PQS_OP_CZ || d.kind == PQS_OP_RZZ;
-        const bool dg1 = d.kind == PQS_OP_CZ || d.kind == PQS_OP_RZZ;
-        if (d.g0 >= 0) st[size_t(d.g0)] = std::max(st[size_t(d.g0)], dg0 ? 1 : 2);
-        if (d.g1 >= 0) st[size_t(d.g1)] = std::max(st[size_t(d.g1)], dg1 ? 1 : 2);
-      }
-      emit_prod = nf > 0;
-    }
-  std::vector<int> modes{0};
-  if (emit_pair) modes.push_back(1);
-  if (emit_prod) modes.push_back(2);
-  for (int mode : modes) {
-    const bool pair = mode == 1, prod = mode == 2;
+  for (int pm = 0; pm < (emit_pair ? 2 : 1); ++pm) {
+    const bool pair = pm == 1;
     // Block b processes flat tile b (flat tile = instance * n_tiles + tile).
     // Persistent form (PQS_PLAN_JIT_PERSISTENT): block b processes tiles b,
     // b + G, b + 2G, ... and issues the loads of its next tile into
     // registers before the current tile's phases, so HBM traffic overlaps
     // compute when only one block fits per CU.
-    o << "extern \"C\" __global__ __launch_bounds__(" << nthr << ") void "
-      << (pair ? "pqs_jit_pair_" : prod ? "pqs_jit_prod_" : "pqs_jit_sweep_") << s
-      << "(const InstDev *__restrict__ insts, long long n_active, const V *__restrict__ init_tab"
+    o << "extern \"C\" __global__ __launch_bounds__(" << nthr << ") void " << (pair ? "pqs_jit_pair_" : "pqs_jit_sweep_")
+      << s << "(const InstDev *__restrict__ insts, long long n_active, const V *__restrict__ init_tab"
       << (pair ? ", double2 *__restrict__ pov" : "") << ") {\n";
     o << "  __shared__ V tile[" << S << "];\n";
     o << "  __shared__ double2 tf[" << (na > 0 ? na : 1) << "];\n";
-    // prod: the 2-vector (amplitude of bit 0, of bit 1) of the qubit at each
-    // global bit position
-    if (prod) o << "  __shared__ V sq[" << 2 * p.n << "];\n";
     o << "  const int tid = threadIdx.x;\n";
     if (pair) {
       o << "  __shared__ double2 pred[" << std::max(1, nthr / 64) << "];\n";
@@ -414,9 +366,7 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     o << "        for (int q = 0; q < " << p.n << "; ++q) v = cmul(v, init_tab[2 * q + int((gi >> (" << (p.n - 1)
       << " - q)) & 1ull)]);\n";
     o << "        pre[m] = v;\n      }\n    }\n  };\n";
-    // prod: no input to fetch; the registers are generated after the
-    // 2-vectors are formed
-    o << (prod ? "  V pre[NA];\n" : "  V pre[NA];\n  fetch(t, pre);\n");
+    o << "  V pre[NA];\n  fetch(t, pre);\n";
     // the final amplitude w of register / slot m: kept (first instance) or
     // paired with the kept one (second instance)
     auto capture = [&](const std::string &m, const std::string &w) {
@@ -437,7 +387,7 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     o << "  const u64 dl = base | dtid;\n";
     o << "  const double *params = in.params;\n";
     o << "  __syncthreads();\n";
-    if (!dload && !prod)
+    if (!dload)
       o << "#pragma unroll\n  for (int m = 0; m < NA; ++m) tile[swz(tid + m * " << nthr << ")] = pre[m];\n";
     if (persist) o << "  if (t + gridDim.x < total) fetch(t + gridDim.x, pre);\n";
     if (na > 0) {
@@ -449,67 +399,7 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         << dbl(cos_min(p)) << ";\n";
       o << "    tf[i] = double2{sn / cs, cs};\n  }\n";
     }
-    if (prod) {
-      // thread g forms the 2-vector of the qubit at global bit position g
-      // (qubit n-1-g): the start state's factor, then its folded gates in
-      // order as unitaries, with (sin, cos) = (tan cos, cos) of the angle
-      // table (the values the sweep's scaled form would apply).  The cases
-      // diverge across lanes, so they hold no transcendental calls.
-      o << "  __syncthreads();\n";
-      o << "  if (tid < " << p.n << ") {\n";
-      o << "    V s0 = V{R(1), R(0)}, s1 = V{R(0), R(0)};\n";
-      o << "    if (in.init == 2) {\n      s0 = init_tab[2 * (" << p.n - 1 << " - tid)];\n      s1 = init_tab[2 * ("
-        << p.n - 1 << " - tid) + 1];\n    }\n";
-      o << "    switch (tid) {\n";
-      for (int g = 0; g < p.n; ++g) {
-        bool any = false;
-        for (int i = ds.op_begin; i < ds.op_end; ++i) {
-          const DevOp &d = p.dev_ops[i];
-          if (!folded[size_t(i - ds.op_begin)] || d.g0 != g) continue;
-          if (!any) o << "    case " << g << ":\n";
-          any = true;
-          const int ai = ang[size_t(i - ds.op_begin)];
-          if (d.kind == PQS_OP_MAT1) {
-            auto cv = [&](int e) { return "V{R(" + dbl(d.m[2 * e]) + "), R(" + dbl(d.m[2 * e + 1]) + ")}"; };
-            o << "      { const V x = s0, y = s1; s0 = cadd(cmul(" << cv(0) << ", x), cmul(" << cv(1)
-              << ", y)); s1 = cadd(cmul(" << cv(2) << ", x), cmul(" << cv(3) << ", y)); }\n";
-            continue;
-          }
-          o << "      { const double2 tc = tf[" << ai << "];\n"
-            << "        const R c = R(tc.y), sv = R(tc.x * tc.y); const V x = s0, y = s1;\n";
-          if (d.kind == PQS_OP_RY)  // [[c, -s], [s, c]]
-            o << "        s0 = V{c * x.x - sv * y.x, c * x.y - sv * y.y}; s1 = V{sv * x.x + c * y.x, sv * x.y + c * y.y}; }\n";
-          else if (d.kind == PQS_OP_RX)  // [[c, -i s], [-i s, c]]
-            o << "        s0 = V{c * x.x + sv * y.y, c * x.y - sv * y.x}; s1 = V{c * y.x + sv * x.y, c * y.y - sv * x.x}; }\n";
-          else  // R_z: diag(c - i s, c + i s)
-            o << "        s0 = V{c * x.x + sv * x.y, c * x.y - sv * x.x}; s1 = V{c * y.x - sv * y.y, c * y.y + sv * y.x}; }\n";
-        }
-        if (any) o << "      break;\n";
-      }
-      o << "    default:\n      break;\n    }\n";
-      o << "    sq[2 * tid] = s0;\n    sq[2 * tid + 1] = s1;\n  }\n";
-    }
     o << "  __syncthreads();\n";
-    if (prod) {
-      // amplitude of global index j = product over positions g of
-      // sq[2 g + bit g of j]: the positions other than the register bits
-      // give one factor per thread, the register bits a product tree
-      o << "  {\n    const u64 gl = base | ltid;\n    V c = V{R(1), R(0)};\n";
-      for (int g = 0; g < p.n; ++g) {
-        bool reg = false;
-        for (int b = 0; b < kRegBits; ++b) reg = reg || lreg[b] == g;
-        if (!reg) o << "    c = cmul(c, sq[" << 2 * g << " + int((gl >> " << g << ") & 1ull)]);\n";
-      }
-      o << "    pre[0] = c;\n";
-      for (int b = 0; b < kRegBits; ++b)
-        for (int m = 0; m < (1 << b); ++m)
-          o << "    pre[" << (m | (1 << b)) << "] = cmul(pre[" << m << "], sq[" << 2 * lreg[b] + 1 << "]);\n"
-            << "    pre[" << m << "] = cmul(pre[" << m << "], sq[" << 2 * lreg[b] << "]);\n";
-      o << "  }\n";
-      if (!dload)
-        o << "#pragma unroll\n  for (int m = 0; m < NA; ++m) tile[swz(tid + m * " << nthr << ")] = pre[m];\n"
-          << "  __syncthreads();\n";
-    }
     // fs: the accumulated scale of the rotations (uniform over the block
     // unless a phase flips its sign per lane); it carries across phases and
     // is applied at the end of the sweep, at the end of a phase with
@@ -561,7 +451,6 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         }
       };
       for (int i = P.op_begin; i < P.op_end; ++i) {
-        if (prod && folded[size_t(i - ds.op_begin)]) continue;  // applied to the 2-vectors
         const DevOp &d = p.dev_ops[i];
         const int ai = ang[i - ds.op_begin];
         const bool r0 = d.r0 >= 0, r1 = d.r1 >= 0;

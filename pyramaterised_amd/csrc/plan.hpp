@@ -145,7 +145,6 @@ struct pqs_plan {
   std::vector<void *> jit_modules;   // one module per specialised sweep
   std::vector<void *> jit_fn;        // per sweep; null: interpreted
   void *jit_pair_fn = nullptr;       // pair kernel of the last sweep (pqs_run_batch_pairs)
-  void *jit_prod_fn = nullptr;       // product-start kernel of sweep 0 (runs from |0> / product tables)
 };
 
 namespace pqs {

@@ -477,10 +477,8 @@ def test_window_search_never_adds_sweeps(pyqc):
 def _phase_counts(plan):
     """Register phases per sweep, counted in the specialised sweep kernel's
     source (each register phase declares its pending-flip flags once; the
-    variants of the first and last sweep in the same source repeat them)."""
-    def plain(src):
-        return [k for k in src.decode().split('extern "C"') if "void pqs_jit_sweep_" in k][0]
-    return [plain(src).count("int xf0 = 0") for src in plan.jit_sweep_sources()]
+    last sweep's pair kernel, which repeats them, follows it)."""
+    return [src.decode().split("void pqs_jit_pair_")[0].count("int xf0 = 0") for src in plan.jit_sweep_sources()]
 
 
 def test_phase_search_never_adds_phases(pyqc):
