@@ -83,6 +83,23 @@ def test_pairs_readback_paths_and_identical_rows(pyqc):
         assert abs(ov[1] - 1.0) < 1e-12, ov[1]
 
 
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-12), ("complex64", 2e-5)])
+def test_pairs_single_sweep_plan(pyqc, dtype, tol):
+    """A circuit that fits one tile: the pair kernel is also the first sweep
+    (it generates |0...0> itself)."""
+    n, p, S = 10, 5, 6
+    qc = _hea(pyqc, n, p, dtype=dtype, jit=True, tile_bits=10, low_bits=3)
+    plan = qc.plan(batch=S)
+    assert plan.n_sweeps == 1
+    og = OC.clifford_he(p, n)
+    params = np.random.default_rng(74).random((S, qc.n_true_params)) * 2 * np.pi
+    st, ov = _check(qc, params, 1e-14 if dtype == "complex128" else 1e-6, 1e-13 if dtype == "complex128" else 1e-6,
+                    fused=True)
+    for i in range(0, S, 2):
+        want = np.vdot(OS.run(og, params[i], n), OS.run(og, params[i + 1], n))
+        assert abs(ov[i // 2] - want) < tol
+
+
 def test_pairs_initial_states_and_errors(pyqc):
     """Product (|+>) and vector initial states through the fused last sweep;
     an odd batch is refused."""
