@@ -407,11 +407,38 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
     o << "  double fs = 1.0;\n";
     // registers carried into a phase that takes its layout by an exchange
     o << "  V areg[NA];\n";
+    // (tan, cos) of the next phase's first rotations, read from the angle
+    // table before this phase's LDS writes: LDS returns in issue order, so
+    // a table read issued after the transition's 16 reads would hold the
+    // next phase's first products until the whole transition has arrived
+    constexpr int kPre = 4;
+    o << "  double2 nxc[" << kPre << "];\n";
+    std::vector<int> pre_ai;  // angle indices held in nxc for this phase
+    auto aexpr = [&](int ai) {
+      for (size_t j = 0; j < pre_ai.size(); ++j)
+        if (pre_ai[j] == ai) return "nxc[" + std::to_string(j) + "]";
+      return "tf[" + std::to_string(ai) + "]";
+    };
+    // emit the reads of the next register phase's first rotations
+    auto preload = [&](int next) -> std::vector<int> {
+      std::vector<int> v;
+      if (next >= ds.phase_end) return v;
+      const DevPhase &Q = p.dev_phases[next];
+      for (int i = Q.op_begin; i < Q.op_end && int(v.size()) < kPre; ++i) {
+        const int ai = ang[i - ds.op_begin];
+        if (ai >= 0) v.push_back(ai);
+      }
+      for (size_t j = 0; j < v.size(); ++j) o << "    nxc[" << j << "] = tf[" << v[j] << "];\n";
+      return v;
+    };
+    std::vector<int> next_pre;
     bool carry = false;
     int nrot = 0;  // rotations since the last check of the accumulated scale
     // phases
     for (int ph = ds.phase_begin; ph < ds.phase_end; ++ph) {
       const DevPhase &P = p.dev_phases[ph];
+      pre_ai.swap(next_pre);
+      next_pre.clear();
       bool lane_sign = false;  // this phase flips fs per lane
       o << "  {\n";
       o << "    int ptid = tid;\n    asm volatile(\"\" : \"+v\"(ptid));\n";
@@ -467,18 +494,18 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         };
         switch (d.kind) {
           case PQS_OP_RY:
-            o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; ry<" << d.r0 << ">(a, "
+            o << "    { const double2 c = " << aexpr(ai) << "; fs *= c.y; ry<" << d.r0 << ">(a, "
               << tsign(fl[d.r0] ? xf(d.r0) : "", "R(c.x)") << "); }\n";
             break;
           case PQS_OP_RX:
-            o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rx<" << d.r0 << ">(a, R(c.x)); }\n";
+            o << "    { const double2 c = " << aexpr(ai) << "; fs *= c.y; rx<" << d.r0 << ">(a, R(c.x)); }\n";
             break;
           case PQS_OP_RZ:
             if (r0)
-              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rz<" << d.r0 << ">(a, "
+              o << "    { const double2 c = " << aexpr(ai) << "; fs *= c.y; rz<" << d.r0 << ">(a, "
                 << tsign(fl[d.r0] ? xf(d.r0) : "", "R(c.x)") << "); }\n";
             else
-              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; pall(a, " << cbit_expr(d, 0)
+              o << "    { const double2 c = " << aexpr(ai) << "; fs *= c.y; pall(a, " << cbit_expr(d, 0)
                 << " ? R(-c.x) : R(c.x)); }\n";
             break;
           case PQS_OP_RZZ: {
@@ -486,14 +513,14 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
             if (r0 && fl[d.r0]) flip = xf(d.r0);
             if (r1 && fl[d.r1]) flip = flip.empty() ? xf(d.r1) : "(" + flip + " ^ " + xf(d.r1) + ")";
             if (r0 && r1)
-              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rzz<" << d.r0 << "," << d.r1 << ">(a, "
+              o << "    { const double2 c = " << aexpr(ai) << "; fs *= c.y; rzz<" << d.r0 << "," << d.r1 << ">(a, "
                 << tsign(flip, "R(c.x)") << "); }\n";
             else if (r0 || r1) {
               const std::string cb = cbit_expr(d, r0 ? 1 : 0);
-              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; rz<" << (r0 ? d.r0 : d.r1) << ">(a, ("
+              o << "    { const double2 c = " << aexpr(ai) << "; fs *= c.y; rz<" << (r0 ? d.r0 : d.r1) << ">(a, ("
                 << (flip.empty() ? cb : "(" + cb + " ^ " + flip + ")") << ") ? R(-c.x) : R(c.x)); }\n";
             } else
-              o << "    { const double2 c = tf[" << ai << "]; fs *= c.y; pall(a, (" << cbit_expr(d, 0) << " ^ "
+              o << "    { const double2 c = " << aexpr(ai) << "; fs *= c.y; pall(a, (" << cbit_expr(d, 0) << " ^ "
                 << cbit_expr(d, 1) << ") ? R(-c.x) : R(c.x)); }\n";
             break;
           }
@@ -575,6 +602,7 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
           o << "    rscale(a, R(fs));\n    fs = 1.0;\n";
           nrot = 0;
         }
+        next_pre = preload(ph + 1);
         const DevPhase &Q = p.dev_phases[ph + 1];
         for (int j = 0; j < Q.xchg; ++j)
           for (int v = 0; v < NA_(); ++v)
@@ -630,6 +658,7 @@ static void emit_sweep(const pqs_plan &p, int s, std::ostringstream &o) {
         o << "  }\n";
         continue;
       }
+      next_pre = preload(ph + 1);
       o << "    const int pstb = pst * int(sizeof(V));\n";
       for (int v = 0; v < NA_(); ++v)
         o << "    LDSB(pstb ^ (" << kv[v] << " * int(sizeof(V)))) = a[" << v << "];\n";
