@@ -359,3 +359,56 @@ def test_find_overparam_point_and_eff_H(pyqc):
     ref = scipy.optimize.minimize(lambda d, F: OM.expr(F, d), [4], args=(Fs,), method="BFGS")
     assert ref.success and abs(x - ref.x[0]) < 1e-6, (x, ref.x[0])
     assert abs(x - N) < 1.0, x
+
+
+def test_sampling_measures_replayed_with_the_oracle(pyqc):
+    """expressibility, entanglement and entropy_of_magic (measure.py:182-197,
+    239-249, 351-359) draw their states from gates.rng as successive
+    run("random") calls would; replayed with the oracle on the same stream
+    they agree.  theta_to_magic / theta_to_gkp (measure.py:251-266) against
+    Pauli-string brute force; get_eigenvalues / new_measure (measure.py:73-99)
+    on the GPU QFIM against the oracle's; the two setters."""
+    from pyramaterised_amd import gates as G
+    from pyramaterised_amd import templates as T
+    n, p = 4, 2
+    qc = pyqc.PQC(n)
+    for l in T.clifford_HE_layers(p, n):
+        qc.add_layer(l)
+    og = OC.clifford_he(p, n)
+    P = qc.n_true_params
+    m = pyqc.measure.Measurements(qc)
+
+    def replay(seed, S):
+        rng = np.random.default_rng(seed)
+        return [OS.run(og, np.array([rng.random(1)[0] * 2 * np.pi for _ in range(P)]), n) for _ in range(S)]
+
+    saved = G.rng
+    try:
+        G.rng = np.random.default_rng(11)
+        ex = m.expressibility(40)
+        G.rng = np.random.default_rng(12)
+        ent = m.entanglement(10)
+        G.rng = np.random.default_rng(13)
+        eom = m.entropy_of_magic(5)
+    finally:
+        G.rng = saved
+    states = replay(11, 40)
+    assert abs(ex - OM.expr(OM.pair_fidelities(states), 2 ** n)) < 1e-9
+    assert np.max(np.abs(np.array(ent) - [OM.single_Q(s, n) for s in replay(12, 10)])) < 1e-12
+    assert abs(eom - np.mean([sre_bruteforce(s, n, 2) for s in replay(13, 5)])) < 1e-10
+    th = list(np.random.default_rng(14).random(P) * 2 * np.pi)
+    ref = OS.run(og, np.array(th), n)
+    assert abs(m.theta_to_magic(th) + sre_bruteforce(ref, n, 2)) < 1e-10
+    assert abs(m.theta_to_gkp(th) + sre_bruteforce(ref, n, 0.5) / (2 * np.log(2))) < 1e-10
+    qc.state = qc.run(th)
+    F = m.get_QFI()
+    Fref = OM.qfim(OS.derivative_states(og, np.array(th), n), ref)
+    ev, _ = m.get_eigenvalues(F)
+    assert np.max(np.abs(ev - np.linalg.eigvalsh(Fref))) < 1e-9
+    assert abs(m.new_measure(F) - OM.new_measure(Fref)) < 1e-9
+    f = lambda x: 0.0  # noqa: E731
+    m.set_minimise_function(f)
+    assert m.minimize_function is f
+    mats = m.get_conversion_matrices()
+    m.set_converstion_matrices(mats)
+    assert m.conversion_matrices is mats
