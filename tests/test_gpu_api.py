@@ -412,3 +412,34 @@ def test_sampling_measures_replayed_with_the_oracle(pyqc):
     mats = m.get_conversion_matrices()
     m.set_converstion_matrices(mats)
     assert m.conversion_matrices is mats
+
+
+def test_pqc_layer_state_and_repr_methods(pyqc):
+    """set_layer / get_layer (circuit.py:45-51) replace a layer, as the
+    oracle's gate list with that layer replaced; update_state and fidelity
+    (circuit.py:127-142); flip_deriv (circuit.py:144-147) leaves every
+    derivative state as it was (the reference flips ``pauli`` after the
+    operators are built); __repr__ (circuit.py:194-198) lists the layers."""
+    from pyramaterised_amd import templates as T
+    n = 4
+    qc = pyqc.PQC(n)
+    for l in T.clifford_HE_layers(2, n):
+        qc.add_layer(l)
+    ry_layer = [pyqc.R_y(q, n) for q in range(n)]
+    qc.set_layer(ry_layer, 1)
+    assert qc.get_layer(1) is not ry_layer and len(qc.get_layer(1)) == n  # a deep copy, as the reference
+    og = OC.clifford_he(1, n) + [OC.R_y(q, n) for q in range(n)]
+    th = np.random.default_rng(21).random(qc.n_true_params) * 2 * np.pi
+    st = qc.update_state(list(th))
+    assert qc.state is st
+    ref = OS.run(og, th, n)
+    assert infidelity(st.full().ravel(), ref) < 1e-12
+    other = OS.run(og, th[::-1].copy(), n)
+    assert abs(qc.fidelity(pyqc.qobj.to_ket(other, n)) - abs(np.vdot(ref, other)) ** 2) < 1e-12
+    _, J0 = qc.jacobian()
+    qc.flip_deriv()
+    _, J1 = qc.jacobian()
+    assert (J0 - J1).abs().max().item() == 0.0
+    r = repr(qc)
+    assert r.startswith(f"A {n} qubit, {qc.n_layers} layer deep PQC. \n")
+    assert r.count("Layer ") == qc.n_layers and "R_y(" in r.split("Layer 1:")[1]
