@@ -71,3 +71,39 @@ def test_qobj_data_view_matches_qutip_accessors():
     assert d.nnz == 2 and d.shape == (2, 2)
     csr = d.tocsr()
     assert csr.nnz == 2 and np.array_equal(csr.toarray(), op.full())
+
+
+def test_clifford_circuit_layers_structure():
+    """templates.py:17-48.  In 'random' mode each of the N draws from
+    [H, S, CZ, CNOT] appends that gate: H and S (PRot subclasses,
+    gates.py:211,286) on a random.randint qubit, CZ / CNOT (EntGate) on a
+    random.sample pair; replaying Python's ``random`` with the same seed gives
+    the same gates on the same qubits.  The other mode puts LEN_CLIFF_STRING
+    H / S gates on each of N // 2 distinct qubits, then a CNOT CHAIN."""
+    import random
+    from pyramaterised_amd import templates as T
+    n, p = 6, 4
+    random.seed(17)
+    layers = T.clifford_circuit_layers(p, n)
+    random.seed(17)
+    want = []
+    for _ in range(p):
+        layer = []
+        for _g in range(n):
+            name = random.choice(["H", "S", "CZ", "CNOT"])
+            if name in ("H", "S"):
+                layer.append((name, random.randint(0, n - 1)))
+            else:
+                layer.append((name, tuple(random.sample(range(n), k=2))))
+        want.append(layer)
+    got = [[(type(g).__name__, g.q_on if type(g).__name__ in ("H", "S") else (g.q1, g.q2)) for g in layer]
+           for layer in layers]
+    assert got == want
+    random.seed(18)
+    other = T.clifford_circuit_layers(2, n, method="strings")
+    for layer in other:
+        assert type(layer[-1]).__name__ == "CHAIN"
+        singles = layer[:-1]
+        assert len(singles) == (n // 2) * T.LEN_CLIFF_STRING
+        assert {type(g).__name__ for g in singles} <= {"H", "S"}
+        assert len({g.q_on for g in singles}) == n // 2
