@@ -328,10 +328,11 @@ int pqs_gram(pqs_ctx *ctx, int32_t dtype, const void *a_dev, int64_t lda, int64_
 /* QFIM from a Jacobian and the state: F_pq = 4 Re(<J_p|J_q> - <J_p|psi><psi|J_q>)
  * into f_dev (P x P float64).  Replaces Measurements.get_QFI
  * (measure.py:33-71).  complex128 forms Re(J J^dag) = R R^T, R = J viewed as a
- * P x 2^(N+1) real matrix, over the upper 128 x 128 tiles on the FP64 matrix
- * cores, split over K (each K chunk's tiles on one XCD, sharing its L2);
- * complex64 Jacobians take the same path with their values widened to
- * double in registers. */
+ * P x 2^(N+1) real matrix, one 64 x 64 upper tile per wave on the FP64 matrix
+ * cores, split over K (each K chunk's tiles on one XCD, sharing its L2), the
+ * diagonal tiles also forming <J_p|psi>; a Jacobian above 1 GiB goes through
+ * in column blocks copied to a padded-row buffer; complex64 Jacobians take
+ * the same path with their values widened to double in registers. */
 int pqs_qfim(pqs_ctx *ctx, int32_t dtype, const void *jac_dev, int64_t P,
              const void *psi_dev, int64_t dim, double *f_dev, void *stream);
 
