@@ -168,6 +168,24 @@ int pqs_ctx_create(int device, pqs_ctx **out);
 int pqs_ctx_destroy(pqs_ctx *ctx);
 int pqs_device_info(pqs_ctx *ctx, int32_t *n_cu, int32_t *max_lds_bytes, char *arch, int32_t arch_len);
 
+/* Cap the blocks (workgroups) of one sweep launch on this context (0, the
+ * default: no cap).  A batched sweep above the cap is launched over
+ * instance ranges of at most `max_blocks` blocks each, with the same
+ * results.  The blocks of a running launch take every slot that frees up
+ * before a kernel of another stream gets one, so a cap keeps a long batched
+ * run (a Jacobian of hundreds of rows) from holding back short kernels of
+ * other streams for its whole length.  No reference counterpart: the
+ * serving loop sets it on the contexts of its QFIM streams. */
+int pqs_ctx_set_max_blocks(pqs_ctx *ctx, int64_t max_blocks);
+
+/* Copy `bytes` host bytes at `src` to device memory at `dst` in the order
+ * of `stream`.  The bytes are staged in the context's pinned buffers before
+ * the call returns (the caller may reuse `src` at once), and a kernel on
+ * `stream` copies them: the host does not wait for the stream's earlier
+ * work, and the copy does not queue behind other streams' DMA copies.  No
+ * reference counterpart: the bindings upload parameters with it. */
+int pqs_upload(pqs_ctx *ctx, void *dst, const void *src, int64_t bytes, void *stream);
+
 /* Compile a flat op list into sweeps (host only; no device needed).
  * Replaces the per-gate operator rebuild and the Python loop
  * ``for g in self.gates: circuit_state = g * circuit_state``

@@ -52,6 +52,8 @@ SIGNATURES = {
     "pqs_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
     "pqs_ctx_destroy": (C.c_int, [vp]),
     "pqs_device_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.c_char_p, i32]),
+    "pqs_upload": (C.c_int, [vp, vp, vp, i64, vp]),
+    "pqs_ctx_set_max_blocks": (C.c_int, [vp, i64]),
     "pqs_plan_create": (C.c_int, [i32, C.POINTER(Op), i32, C.POINTER(PlanOpts), C.POINTER(vp)]),
     "pqs_plan_destroy": (C.c_int, [vp]),
     "pqs_plan_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),

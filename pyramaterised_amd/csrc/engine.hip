@@ -1904,10 +1904,23 @@ struct pqs_ctx {
   hipStream_t ws_stream = nullptr;  // stream of the last call that used ws
   hipEvent_t ws_event = nullptr;
   void *scratch = nullptr;  // this device's address of the file-scope scratch array
-  void *pin = nullptr;
-  size_t pin_bytes = 0;
-  hipEvent_t pin_event = nullptr;
-  bool pin_pending = false;
+  // Pinned staging for host-to-device uploads: a ring of buffers, each
+  // reused only once its own last copy has run.  A busy buffer is never
+  // waited for while the ring may still grow (up to kPinMax buffers), so an
+  // upload does not hold the host thread behind the stream's earlier
+  // kernels.  The copy itself is a kernel on the caller's stream that reads
+  // the pinned buffer: a DMA-engine copy would queue behind other streams'
+  // copies, which wait for those streams' kernels.
+  struct PinSlot {
+    void *host = nullptr;
+    size_t bytes = 0;
+    hipEvent_t event = nullptr;
+    bool pending = false;
+  };
+  static constexpr int kPinMax = 32;
+  std::vector<PinSlot> pins;
+  int pin_next = 0;
+  int64_t max_blocks = 0;  // cap on the blocks of one sweep launch (0: none)
 };
 
 namespace {
@@ -1954,50 +1967,131 @@ int ws_reserve(pqs_ctx *ctx, size_t bytes, hipStream_t st) {
   return PQS_OK;
 }
 
-// Copy host bytes to device memory asynchronously through pinned staging.
+__global__ void k_upload16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void k_upload1(const unsigned char *__restrict__ src, unsigned char *__restrict__ dst, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+// true once the event's work has run; false while it is pending (the
+// not-ready status is cleared so that a later launch check does not see it)
+bool event_done(hipEvent_t ev) {
+  const hipError_t e = hipEventQuery(ev);
+  if (e == hipSuccess) return true;
+  (void)hipGetLastError();
+  return false;
+}
+
+// Copy host bytes to device memory in stream order: into a pinned staging
+// buffer now, then by a copy kernel on `st`, which reads the pinned buffer.
 int upload(pqs_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t st) {
   if (bytes == 0) return PQS_OK;
-  if (ctx->pin_pending) {
-    HIPCHECK(hipEventSynchronize(ctx->pin_event));
-    ctx->pin_pending = false;
+  auto &pins = ctx->pins;
+  const int n_slots = int(pins.size());
+  int k = -1;
+  for (int i = 0; i < n_slots && k < 0; ++i) {
+    const int j = (ctx->pin_next + i) % n_slots;
+    pqs_ctx::PinSlot &sl = pins[size_t(j)];
+    if (sl.pending && event_done(sl.event)) sl.pending = false;
+    if (!sl.pending && sl.bytes >= bytes) k = j;
   }
-  if (bytes > ctx->pin_bytes) {
-    if (ctx->pin) HIPCHECK(hipHostFree(ctx->pin));
-    ctx->pin = nullptr;
-    size_t nb = std::max<size_t>(bytes, size_t(1) << 20);
-    HIPCHECK(hipHostMalloc(&ctx->pin, nb, hipHostMallocDefault));
-    ctx->pin_bytes = nb;
+  if (k < 0 && n_slots < pqs_ctx::kPinMax) {
+    pqs_ctx::PinSlot sl;
+    HIPCHECK(hipEventCreateWithFlags(&sl.event, hipEventDisableTiming));
+    const size_t nb = std::max<size_t>((bytes + 4095) & ~size_t(4095), size_t(1) << 20);
+    const hipError_t e = hipHostMalloc(&sl.host, nb, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      (void)hipEventDestroy(sl.event);
+      return hip_fail(e, "hipHostMalloc (upload staging)");
+    }
+    sl.bytes = nb;
+    pins.push_back(sl);
+    k = n_slots;
   }
-  std::memcpy(ctx->pin, src, bytes);
-  HIPCHECK(hipMemcpyAsync(dst, ctx->pin, bytes, hipMemcpyHostToDevice, st));
-  HIPCHECK(hipEventRecord(ctx->pin_event, st));
-  ctx->pin_pending = true;
+  if (k < 0) {  // every buffer busy or too small and the ring is full: wait for the next one
+    k = ctx->pin_next % n_slots;
+    pqs_ctx::PinSlot &sl = pins[size_t(k)];
+    if (sl.pending) HIPCHECK(hipEventSynchronize(sl.event));
+    sl.pending = false;
+    if (sl.bytes < bytes) {
+      HIPCHECK(hipHostFree(sl.host));
+      sl.host = nullptr;
+      sl.bytes = 0;
+      const size_t nb = (bytes + 4095) & ~size_t(4095);
+      HIPCHECK(hipHostMalloc(&sl.host, nb, hipHostMallocDefault));
+      sl.bytes = nb;
+    }
+  }
+  ctx->pin_next = (k + 1) % int(pins.size());
+  pqs_ctx::PinSlot &sl = pins[size_t(k)];
+  std::memcpy(sl.host, src, bytes);
+  const bool wide = (reinterpret_cast<uintptr_t>(dst) & 15) == 0 && (bytes & 15) == 0;
+  const int64_t n = wide ? int64_t(bytes >> 4) : int64_t(bytes);
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+  if (wide)
+    hipLaunchKernelGGL(k_upload16, dim3(unsigned(blocks)), dim3(256), 0, st, (const uint4 *)sl.host, (uint4 *)dst, n);
+  else
+    hipLaunchKernelGGL(k_upload1, dim3(unsigned(blocks)), dim3(256), 0, st, (const unsigned char *)sl.host,
+                       (unsigned char *)dst, n);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(sl.event, st));
+  sl.pending = true;
   return PQS_OK;
 }
 
-int plan_upload(pqs_ctx *ctx, pqs_plan *p) {
-  if (p->d_ops && p->dev_device == ctx->device) return PQS_OK;
-  if (p->d_ops) return fail(PQS_ERR_ARG, "plan already bound to another device");
-  size_t ob = std::max<size_t>(1, p->dev_ops.size()) * sizeof(DevOp);
-  size_t tb = std::max<size_t>(1, p->dev_terms.size()) * sizeof(DevTerm);
-  HIPCHECK(hipMalloc(&p->d_ops, ob));
-  HIPCHECK(hipMalloc(&p->d_terms, tb));
-  if (!p->dev_ops.empty())
-    HIPCHECK(hipMemcpy(p->d_ops, p->dev_ops.data(), p->dev_ops.size() * sizeof(DevOp), hipMemcpyHostToDevice));
-  if (!p->dev_terms.empty())
-    HIPCHECK(hipMemcpy(p->d_terms, p->dev_terms.data(), p->dev_terms.size() * sizeof(DevTerm),
-                       hipMemcpyHostToDevice));
+// A launch on `st` of a plan whose arrays may still be in flight on the
+// stream of its first launch: `st` waits for that copy.
+int plan_wait(pqs_plan *p, hipStream_t st) {
+  if (p->up_pending) {
+    if (event_done((hipEvent_t)p->up_event))
+      p->up_pending = false;
+    else if ((hipStream_t)p->up_stream != st)
+      HIPCHECK(hipStreamWaitEvent(st, (hipEvent_t)p->up_event, 0));
+  }
+  return PQS_OK;
+}
+
+// The plan's arrays on the context's device, copied at its first launch in
+// that launch's stream order (no host wait).
+int plan_upload(pqs_ctx *ctx, pqs_plan *p, hipStream_t st) {
+  if (p->dev_device >= 0 && p->dev_device != ctx->device)
+    return fail(PQS_ERR_ARG, "plan already bound to another device");
+  if (p->d_ops) return plan_wait(p, st);
+  if (!p->up_event) {
+    hipEvent_t ev;
+    HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    p->up_event = (void *)ev;
+  }
+  const size_t ob = std::max<size_t>(1, p->dev_ops.size()) * sizeof(DevOp);
+  const size_t tb = std::max<size_t>(1, p->dev_terms.size()) * sizeof(DevTerm);
   const size_t pb = std::max<size_t>(1, p->dev_phases.size()) * sizeof(pqs::DevPhase);
-  HIPCHECK(hipMalloc(&p->d_phases, pb));
-  if (!p->dev_phases.empty())
-    HIPCHECK(hipMemcpy(p->d_phases, p->dev_phases.data(), p->dev_phases.size() * sizeof(pqs::DevPhase),
-                       hipMemcpyHostToDevice));
   const size_t sb = std::max<size_t>(1, p->dev_sweeps.size()) * sizeof(DevSweep);
-  HIPCHECK(hipMalloc(&p->d_sweeps, sb));
-  if (!p->dev_sweeps.empty())
-    HIPCHECK(hipMemcpy(p->d_sweeps, p->dev_sweeps.data(), p->dev_sweeps.size() * sizeof(DevSweep),
-                       hipMemcpyHostToDevice));
+  void *d[4] = {nullptr, nullptr, nullptr, nullptr};
+  const size_t nb[4] = {ob, tb, pb, sb};
+  for (int i = 0; i < 4; ++i) {
+    const hipError_t e = hipMalloc(&d[i], nb[i]);
+    if (e != hipSuccess) {
+      for (int j = 0; j < i; ++j) (void)hipFree(d[j]);
+      return hip_fail(e, "hipMalloc (plan arrays)");
+    }
+  }
+  p->d_ops = d[0];
+  p->d_terms = d[1];
+  p->d_phases = d[2];
+  p->d_sweeps = d[3];
   p->dev_device = ctx->device;
+  int rc = upload(ctx, p->d_ops, p->dev_ops.data(), p->dev_ops.size() * sizeof(DevOp), st);
+  if (!rc) rc = upload(ctx, p->d_terms, p->dev_terms.data(), p->dev_terms.size() * sizeof(DevTerm), st);
+  if (!rc) rc = upload(ctx, p->d_phases, p->dev_phases.data(), p->dev_phases.size() * sizeof(pqs::DevPhase), st);
+  if (!rc) rc = upload(ctx, p->d_sweeps, p->dev_sweeps.data(), p->dev_sweeps.size() * sizeof(DevSweep), st);
+  if (rc) return rc;
+  HIPCHECK(hipEventRecord((hipEvent_t)p->up_event, st));
+  p->up_stream = (void *)st;
+  p->up_pending = true;
   return PQS_OK;
 }
 
@@ -2110,8 +2204,8 @@ int launch_sweep_reg(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, i
   return PQS_OK;
 }
 
-int launch_sweep(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64_t n_active,
-                 const void *init_tab, hipStream_t st) {
+int launch_sweep_one(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64_t n_active,
+                     const void *init_tab, hipStream_t st) {
   if (p->jit_module && p->dev_device == ctx->device && s < int(p->jit_fn.size()) && p->jit_fn[s]) {
     const DevSweep &sw = p->dev_sweeps[s];
     const int nthr = (1 << sw.k) >> pqs::kRegBits;
@@ -2151,6 +2245,25 @@ int launch_sweep(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64
   }
 }
 
+// One sweep over n_active instances.  With a block cap on the context the
+// launch is split over instance ranges of at most max_blocks blocks each
+// (an instance's tiles stay in one launch): the blocks of a launch that has
+// started take every slot that frees up before a kernel of another stream
+// gets one, so one long launch would hold back other streams' short kernels
+// for its whole length.
+int launch_sweep(pqs_ctx *ctx, pqs_plan *p, int s, const InstDev *d_insts, int64_t n_active,
+                 const void *init_tab, hipStream_t st) {
+  const int64_t tiles = p->dev_sweeps[size_t(s)].n_tiles;
+  if (ctx->max_blocks <= 0 || tiles * n_active <= ctx->max_blocks)
+    return launch_sweep_one(ctx, p, s, d_insts, n_active, init_tab, st);
+  const int64_t per = std::max<int64_t>(1, ctx->max_blocks / std::max<int64_t>(1, tiles));
+  for (int64_t i0 = 0; i0 < n_active; i0 += per) {
+    const int rc = launch_sweep_one(ctx, p, s, d_insts + i0, std::min(per, n_active - i0), init_tab, st);
+    if (rc) return rc;
+  }
+  return PQS_OK;
+}
+
 size_t amp_bytes(int dtype) { return dtype == PQS_C128 ? 16 : 8; }
 
 }  // namespace
@@ -2166,6 +2279,22 @@ int pqs_range_push(const char *name) {
 int pqs_range_pop(void) {
   if (roctx().pop) roctx().pop();
   return PQS_OK;
+}
+
+int pqs_ctx_set_max_blocks(pqs_ctx *ctx, int64_t max_blocks) {
+  if (!ctx) return fail(PQS_ERR_ARG, "ctx is NULL");
+  if (max_blocks < 0) return fail(PQS_ERR_ARG, "negative block cap");
+  ctx->max_blocks = max_blocks;
+  return PQS_OK;
+}
+
+int pqs_upload(pqs_ctx *ctx, void *dst, const void *src, int64_t bytes, void *stream) {
+  if (!ctx) return fail(PQS_ERR_ARG, "ctx is NULL");
+  if (bytes < 0) return fail(PQS_ERR_ARG, "negative byte count");
+  if (bytes == 0) return PQS_OK;
+  if (!dst || !src) return fail(PQS_ERR_ARG, "NULL dst or src");
+  DeviceGuard g(ctx->device);
+  return upload(ctx, dst, src, size_t(bytes), (hipStream_t)stream);
 }
 
 int pqs_ctx_create(int device, pqs_ctx **out) {
@@ -2207,12 +2336,11 @@ int pqs_ctx_create(int device, pqs_ctx **out) {
     }
     c->scratch = device < 64 ? addr[device] : nullptr;
   }
-  if (hipEventCreateWithFlags(&c->pin_event, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ws_event, hipEventDisableTiming) != hipSuccess) {
-    if (c->pin_event) (void)hipEventDestroy(c->pin_event);
+  if (hipEventCreateWithFlags(&c->ws_event, hipEventDisableTiming) != hipSuccess) {
     delete c;
     return fail(PQS_ERR_HIP, "hipEventCreate failed");
   }
+  c->pins.reserve(pqs_ctx::kPinMax);
   *out = c;
   return PQS_OK;
 }
@@ -2225,8 +2353,10 @@ int pqs_ctx_destroy(pqs_ctx *ctx) {
     (void)hipFreeAsync(ctx->ws, ctx->ws_stream);
     (void)hipStreamSynchronize(ctx->ws_stream);
   }
-  if (ctx->pin) (void)hipHostFree(ctx->pin);
-  if (ctx->pin_event) (void)hipEventDestroy(ctx->pin_event);
+  for (pqs_ctx::PinSlot &sl : ctx->pins) {
+    if (sl.host) (void)hipHostFree(sl.host);
+    if (sl.event) (void)hipEventDestroy(sl.event);
+  }
   if (ctx->ws_event) (void)hipEventDestroy(ctx->ws_event);
   delete ctx;
   return PQS_OK;
@@ -2242,7 +2372,7 @@ int pqs_device_info(pqs_ctx *ctx, int32_t *n_cu, int32_t *max_lds, char *arch, i
 
 int pqs_plan_destroy(pqs_plan *p) {
   if (!p) return PQS_OK;
-  if (p->d_ops || p->d_terms) {
+  if (p->dev_device >= 0 && (p->d_ops || !p->jit_modules.empty())) {
     DeviceGuard g(p->dev_device);
     (void)hipDeviceSynchronize();
     if (p->d_ops) (void)hipFree(p->d_ops);
@@ -2251,6 +2381,7 @@ int pqs_plan_destroy(pqs_plan *p) {
     if (p->d_sweeps) (void)hipFree(p->d_sweeps);
     for (void *m : p->jit_modules) (void)hipModuleUnload((hipModule_t)m);
   }
+  if (p->up_event) (void)hipEventDestroy((hipEvent_t)p->up_event);
   delete p;
   return PQS_OK;
 }
@@ -2442,7 +2573,7 @@ int pqs_run(pqs_ctx *ctx, pqs_plan *p, const pqs_instance *insts, int64_t n_inst
     d.pingpong = 0;
     h[size_t(i)] = d;
   }
-  int rc = plan_upload(ctx, p);
+  int rc = plan_upload(ctx, p, st);
   if (rc) return rc;
   return run_instances(ctx, p, h, init_tab, st);
 }
@@ -2474,7 +2605,7 @@ int pqs_run_batch(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_stride,
     d.start_sweep = 0;
     d.pingpong = 0;
   }
-  int rc = plan_upload(ctx, p);
+  int rc = plan_upload(ctx, p, st);
   if (rc) return rc;
   return run_instances(ctx, p, h, init_tab, st);
 }
@@ -2504,7 +2635,7 @@ int pqs_run_batch_pairs(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_s
     d.start_sweep = 0;
     d.pingpong = 0;
   }
-  int rc = plan_upload(ctx, p);
+  int rc = plan_upload(ctx, p, st);
   if (rc) return rc;
   return run_instances(ctx, p, h, init_tab, st, overlaps);
 }
@@ -2522,7 +2653,7 @@ int pqs_run_multi(pqs_ctx *ctx, pqs_plan *const *plans, int32_t n_plans, const p
     pqs_plan *p = plans[k];
     if (!p) return fail(PQS_ERR_ARG, "NULL plan");
     if (p->dtype != dtype) return fail(PQS_ERR_ARG, "plans of one multi-plan run share one dtype");
-    int rc = plan_upload(ctx, p);
+    int rc = plan_upload(ctx, p, st);
     if (rc) return rc;
   }
   const size_t ab = amp_bytes(dtype);
@@ -2702,7 +2833,7 @@ int pqs_batch_table(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state_strid
     d.pingpong = p->rolling ? 1 : 0;
     if (p->rolling && src == d.state) return fail(PQS_ERR_ARG, "tabled rolling run: src must not be the output");
   }
-  int rc = plan_upload(ctx, p);
+  int rc = plan_upload(ctx, p, st);
   if (rc) return rc;
   return upload(ctx, table, h.data(), h.size() * sizeof(InstDev), st);
 }
@@ -2715,6 +2846,7 @@ int pqs_run_table(pqs_ctx *ctx, pqs_plan *p, const void *table, int64_t batch, c
   if (!p->d_ops || p->dev_device != ctx->device) return fail(PQS_ERR_ARG, "plan not on this device: table it first");
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
+  if (int rc = plan_wait(p, st)) return rc;
   for (int s = 0; s < int(p->dev_sweeps.size()); ++s) {
     int rc = launch_sweep(ctx, p, s, (const InstDev *)table, batch, init_tab, st);
     if (rc) return rc;
@@ -2744,6 +2876,7 @@ int pqs_run_batch_noalloc(pqs_ctx *ctx, pqs_plan *p, void *states, int64_t state
   }
   DeviceGuard g(ctx->device);
   hipStream_t st = (hipStream_t)stream;
+  if (int rc = plan_wait(p, st)) return rc;
   InstDev *tab = static_cast<InstDev *>(ctx->scratch);
   const int64_t sbytes = state_stride * int64_t(amp_bytes(p->dtype));
   hipLaunchKernelGGL(k_fill_batch_table, dim3(grid_for(batch, 256)), dim3(256), 0, st, tab, (char *)states, sbytes,
@@ -2769,7 +2902,7 @@ int pqs_jacobian(pqs_ctx *ctx, pqs_plan *p, const double *params, int32_t init, 
   const int S = int(p->dev_sweeps.size());
   const int64_t dim = int64_t(1) << p->n;
   const size_t ab = amp_bytes(p->dtype);
-  int rc = plan_upload(ctx, p);
+  int rc = plan_upload(ctx, p, st);
   if (rc) return rc;
   // Rolling layouts write every sweep out of place, so each tangent row
   // ping-pongs with a workspace row as the primal does; fixed layouts update
@@ -3506,8 +3639,10 @@ int pqs_plan_jit(pqs_ctx *ctx, pqs_plan *p, int32_t load) {
     if (rcs[i]) return fail(PQS_ERR_HIP, "hiprtc compile failed: " + errs[i].substr(0, 2000));
   if (!load) return PQS_OK;
   DeviceGuard g(ctx->device);
-  int rc = plan_upload(ctx, p);
-  if (rc) return rc;
+  // bind the plan to this device; its arrays are copied at the first launch
+  if (p->dev_device >= 0 && p->dev_device != ctx->device)
+    return fail(PQS_ERR_ARG, "plan already bound to another device");
+  p->dev_device = ctx->device;
   std::vector<void *> mods, fns(p->dev_sweeps.size(), nullptr);
   void *pair_fn = nullptr;
   const int last = int(p->dev_sweeps.size()) - 1;

@@ -139,6 +139,11 @@ struct pqs_plan {
   void *d_terms = nullptr;
   void *d_phases = nullptr;
   void *d_sweeps = nullptr;  // DevSweep records (launches that mix plans)
+  // the arrays are copied in stream order on the stream of the plan's first
+  // launch; launches on other streams wait for this event until it has run
+  void *up_event = nullptr;   // hipEvent_t
+  void *up_stream = nullptr;  // hipStream_t
+  bool up_pending = false;
 
   // run-time specialised sweep kernels (hiprtc), per plan on one device
   void *jit_module = nullptr;        // non-null once loaded

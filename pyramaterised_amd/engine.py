@@ -58,6 +58,12 @@ class Context:
         check(L.lib().pqs_device_info(self.handle, C.byref(n_cu), C.byref(lds), arch, 64))
         self.n_cu, self.max_lds, self.arch = n_cu.value, lds.value, arch.value.decode()
 
+    def set_max_blocks(self, max_blocks):
+        """Cap the blocks of one sweep launch (0: no cap); a larger batched
+        sweep is split over instance ranges, with the same results.  Short
+        kernels of other streams then wait at most one such launch."""
+        check(L.lib().pqs_ctx_set_max_blocks(self.handle, int(max_blocks)))
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h and h.value:
@@ -414,10 +420,14 @@ class Init:
         """(init code, src tensor or None, table tensor or None)."""
         if self.kind == "zero":
             return 1, None, None
+        npdt = np.complex128 if CDTYPE[dtype_c] == torch.complex128 else np.complex64
         if self.kind == "product":
-            tab = torch.as_tensor(self.table.reshape(-1), dtype=CDTYPE[dtype_c], device=dev)
-            return 2, None, tab
+            if dev == device():
+                return 2, None, to_device(np.asarray(self.table).reshape(-1), npdt)
+            return 2, None, torch.as_tensor(self.table.reshape(-1), dtype=CDTYPE[dtype_c], device=dev)
         v = self.vector
+        if not torch.is_tensor(v) and dev == device():
+            return 0, to_device(np.asarray(v).reshape(-1), npdt), None
         if not torch.is_tensor(v):
             v = torch.as_tensor(np.asarray(v).reshape(-1))
         v = v.to(device=dev, dtype=CDTYPE[dtype_c]).contiguous()
@@ -426,9 +436,34 @@ class Init:
 
 # ------------------------------------------------------------------ entry points
 
+_NP_TO_TORCH = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+                np.dtype(np.complex128): torch.complex128, np.dtype(np.complex64): torch.complex64,
+                np.dtype(np.int64): torch.int64, np.dtype(np.int32): torch.int32}
+
+
+def to_device(a, dtype=None):
+    """Host array -> device tensor on the current device, in the current
+    stream's order.  The bytes are staged in the context's pinned buffers
+    before this returns and a kernel on the stream copies them (pqs_upload),
+    so the host does not wait for the stream's earlier work and the copy
+    does not queue behind other streams' DMA copies."""
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.dtype not in _NP_TO_TORCH:
+        raise TypeError(f"to_device: unsupported dtype {a.dtype}")
+    t = torch.empty(a.shape, dtype=_NP_TO_TORCH[a.dtype], device=device())
+    if a.nbytes:
+        check(L.lib().pqs_upload(context().handle, ptr(t), C.c_void_p(a.ctypes.data), a.nbytes, stream_ptr()))
+    return t
+
+
 def _params_tensor(params, dev, rows=None):
-    p = params if torch.is_tensor(params) else torch.as_tensor(np.asarray(params, dtype=np.float64))
-    p = p.to(device=dev, dtype=torch.float64).contiguous()
+    if torch.is_tensor(params) and (params.is_cuda or params.requires_grad):
+        p = params.to(device=dev, dtype=torch.float64).contiguous()
+    else:
+        host = params.to(torch.float64).numpy() if torch.is_tensor(params) else params
+        if dev != device():
+            raise ValueError("parameters on the host go to the current device only")
+        p = to_device(np.asarray(host, dtype=np.float64))
     if rows is not None and p.dim() == 1:
         p = p.unsqueeze(0)
     return p

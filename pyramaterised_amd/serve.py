@@ -158,12 +158,6 @@ def make_workload(n_requests, rate, seed=0, n_range=(6, 20), depth_range=(2, 40)
     return out
 
 
-def _to_device(a, dev):
-    """Host array -> device tensor through pinned memory, asynchronous on the
-    current stream (a pageable copy would block the host until the stream's
-    earlier work is done)."""
-    return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dev, non_blocking=True)
-
 
 class _Circuit:
     """Compiled plans of one (n, depth) class, shared by its requests.
@@ -244,16 +238,20 @@ class Server:
 
     hbm_budget: device bytes that admitted, unreleased requests may hold
     (default: half of the device memory; each stream also keeps one Jacobian
-    buffer, sized to the largest QFIM it served, until ``close``).  n_streams: concurrent HIP streams
-    (the first one carries large requests, one at a time; large_bytes is
-    the device footprint from which a QFIM request counts as large).
+    buffer, sized to the largest QFIM it served, until ``close``).  n_streams: concurrent HIP streams;
+    QFIMs and state requests do not share one (with three or more, the
+    first carries large QFIMs one at a time, the second the other QFIMs and
+    the rest the state requests; large_bytes is the device footprint from
+    which a QFIM request counts as large; qfim_launch_blocks caps the blocks
+    of one sweep launch on the QFIM streams, so state kernels do not wait
+    behind whole Jacobian sweeps).
     max_batch: most requests of one class per launch.  keep_states: keep
     every state-request's state resident (counted in the budget) until
     ``release``; otherwise only Q is returned."""
 
-    def __init__(self, hbm_budget=None, n_streams=2, max_batch=512, dtype="complex128", keep_states=True,
+    def __init__(self, hbm_budget=None, n_streams=4, max_batch=512, dtype="complex128", keep_states=True,
                  max_inflight_per_stream=2, specialise_qfim_from=14, specialise_state_from=18, compile_workers=8,
-                 large_bytes=1 << 30, merge_classes=True, prepare_workers=4):
+                 large_bytes=1 << 30, merge_classes=True, prepare_workers=4, qfim_launch_blocks=16384):
         self.dev = E.device()
         if hbm_budget is None:
             _, total = torch.cuda.mem_get_info(self.dev)
@@ -268,6 +266,13 @@ class Server:
         self.large_bytes = int(large_bytes)
         self.merge_classes = bool(merge_classes)
         self.slots = [_Slot(self.dev) for _ in range(max(1, n_streams))]
+        if len(self.slots) > 1:
+            # the streams that carry QFIMs split their Jacobian sweeps into
+            # launches of at most this many blocks: a state batch's kernels
+            # on another stream wait at most one such launch, not a whole
+            # sweep of hundreds of rows (tens of milliseconds each)
+            for s in self.slots[:1 if len(self.slots) == 2 else 2]:
+                s.ctx.set_max_blocks(qfim_launch_blocks)
         self._next_slot = 0
         self.circuits = {}
         self.queues = OrderedDict()  # class key -> deque of requests (FIFO)
@@ -428,22 +433,41 @@ class Server:
         self._n_settled += len(reqs)
 
     # ---------------------------------------------------------------- loop
-    def _free_slot(self, large):
-        """Large launches (they fill the GPU on their own) share the first
-        stream, one at a time, so they neither compete with each other nor
-        scatter their multi-GiB buffers over several streams' allocator
-        pools; small launches round-robin over the other streams."""
-        if large or len(self.slots) == 1:
-            s = self.slots[0]
-            limit = 1 if large else self.max_inflight
+    def _free_slot(self, lane):
+        """A stream with room for a launch of ``lane`` ("large": a QFIM that
+        fills the GPU on its own; "qfim": any other QFIM; "state"), or None.
+        Kinds do not share a stream, so a state batch never waits in stream
+        order behind a Jacobian that takes hundreds of milliseconds:
+        - one stream: everything;
+        - two: QFIMs (large ones one at a time) on the first, states on the second;
+        - three or more: large QFIMs one at a time on the first (they neither
+          compete with each other nor scatter their multi-GiB buffers over
+          several streams' allocator pools), other QFIMs on the second, and
+          states round-robin over the rest."""
+        S = len(self.slots)
+        if S == 1:
+            cands = self.slots
+        elif lane == "state":
+            cands = self.slots[1:] if S == 2 else self.slots[2:]
+        elif lane == "large" or S == 2:
+            cands = self.slots[:1]
+        else:
+            cands = self.slots[1:2]
+        limit = 1 if lane == "large" else self.max_inflight
+        if lane == "large":
+            s = cands[0]
             return s if len(s.inflight) < limit else None
-        others = self.slots[1:]
-        for k in range(len(others)):
-            s = others[(self._next_slot + k) % len(others)]
-            if len(s.inflight) < self.max_inflight:
-                self._next_slot = (self._next_slot + k + 1) % len(others)
+        for k in range(len(cands)):
+            s = cands[(self._next_slot + k) % len(cands)]
+            if len(s.inflight) < limit:
+                self._next_slot = (self._next_slot + k + 1) % len(cands)
                 return s
         return None
+
+    def _lane(self, req):
+        if req.kind != QFIM:
+            return "state"
+        return "large" if self.is_large(req) else "qfim"
 
     def is_large(self, req):
         """A request whose launch alone fills the GPU (a QFIM whose Jacobian
@@ -462,7 +486,7 @@ class Server:
     def _admit_merged(self, now):
         """One multi-plan launch (pqs_run_multi) of the oldest admissible
         state requests of every mergeable class, widths and depths mixed."""
-        slot = self._free_slot(False)
+        slot = self._free_slot("state")
         if slot is None:
             return 0
         take, need = [], 0
@@ -503,7 +527,7 @@ class Server:
             q = self.queues[key]
             if not q or not self._ready(key, now):
                 continue
-            slot = self._free_slot(self.is_large(q[0]))
+            slot = self._free_slot(self._lane(q[0]))
             if slot is None:
                 continue
             q = self.queues[key]
@@ -541,7 +565,7 @@ class Server:
         try:
             with E.using(slot.ctx, slot.stream), E.marked(f"serve {kind} n={n} depth={depth} batch={len(reqs)}"):
                 if kind == STATE_MW:
-                    params = _to_device(np.stack([r.params for r in reqs]), self.dev)
+                    params = E.to_device(np.stack([r.params for r in reqs]), np.float64)
                     b.out = E.run(c.best("plan"), params, c.init)
                     _, b.q = E.mw(b.out, n)
                 else:
@@ -553,7 +577,7 @@ class Server:
                         buf = self._jscratch[id(slot)] = torch.empty(P * dim, dtype=E.CDTYPE[self.dtype],
                                                                      device=self.dev)
                     J = buf[:P * dim].view(P, dim)
-                    psi, J = E.jacobian(c.best("gplan"), _to_device(r.params, self.dev), c.init, jac_out=J)
+                    psi, J = E.jacobian(c.best("gplan"), E.to_device(r.params, np.float64), c.init, jac_out=J)
                     b.F = E.qfim(J, psi)
                     b.out = psi
                     del J
@@ -589,7 +613,7 @@ class Server:
                 jobs = []
                 for (n, depth), rs in groups.items():
                     c = self.circuit(n, depth)
-                    jobs.append((c.plan, _to_device(np.stack([r.params for r in rs]), self.dev), c.init))
+                    jobs.append((c.plan, E.to_device(np.stack([r.params for r in rs]), np.float64), c.init))
                 outs = E.run_multi(jobs)
                 qs = []
                 b.out = []

@@ -53,6 +53,46 @@ def test_request_bytes_counts_rolling_jacobian_rows():
     assert PQC(10).rolling_jacobian() is True
 
 
+def test_stream_lanes_keep_kinds_apart():
+    """QFIMs and state requests never share a stream once there are two;
+    with three or more, large QFIMs run one at a time on the first stream,
+    other QFIMs on the second and states round-robin over the rest."""
+    import types
+    from collections import deque
+
+    def server(n_streams, max_inflight=2):
+        return types.SimpleNamespace(slots=[types.SimpleNamespace(inflight=deque(), i=i) for i in range(n_streams)],
+                                     max_inflight=max_inflight, _next_slot=0)
+
+    pick = SV.Server._free_slot
+    s1 = server(1)
+    assert all(pick(s1, lane) is s1.slots[0] for lane in ("large", "qfim", "state"))
+    s1.slots[0].inflight.append(0)
+    assert pick(s1, "large") is None and pick(s1, "state") is s1.slots[0]
+    s2 = server(2)
+    assert pick(s2, "large").i == 0 and pick(s2, "qfim").i == 0 and pick(s2, "state").i == 1
+    s2.slots[0].inflight.append(0)
+    assert pick(s2, "large") is None and pick(s2, "qfim").i == 0
+    s2.slots[1].inflight.extend([0, 0])
+    assert pick(s2, "state") is None  # full: a state request waits rather than join the QFIM stream
+    s4 = server(4)
+    assert pick(s4, "large").i == 0 and pick(s4, "qfim").i == 1
+    got = [pick(s4, "state").i for _ in range(4)]
+    assert sorted(set(got)) == [2, 3]
+    s4.slots[1].inflight.extend([0, 0])
+    assert pick(s4, "qfim") is None
+    s4.slots[2].inflight.extend([0, 0])
+    s4.slots[3].inflight.extend([0, 0])
+    assert pick(s4, "state") is None and pick(s4, "large").i == 0
+    stub = types.SimpleNamespace(large_bytes=1 << 30, amp=16)
+    stub.is_large = lambda r: SV.Server.is_large(stub, r)
+    stub.request_bytes = lambda r: SV.Server.request_bytes(stub, r)
+    lane = SV.Server._lane
+    assert lane(stub, SV.Request(0, 8, 2, SV.STATE_MW, np.zeros(32), 0.0)) == "state"
+    assert lane(stub, SV.Request(0, 8, 2, SV.QFIM, np.zeros(32), 0.0)) == "qfim"
+    assert lane(stub, SV.Request(0, 20, 40, SV.QFIM, np.zeros(1600), 0.0)) == "large"
+
+
 def test_wire_format_round_trips_bit_exact():
     """Requests and results written as JSON lines read back bit-exact
     (parameters through float repr, arrays through base64 bytes), and a
