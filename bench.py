@@ -432,6 +432,11 @@ def leg_serving(args):
                "one_at_a_time_s": wall, "one_at_a_time_requests_per_s": len(reqs) / wall,
                "max_abs_dQ_mix_vs_alone": dq, "max_rel_dQFIM_mix_vs_alone": dF,
                "qfim_requests": sum(r.kind == SV.QFIM for r in reqs)})
+    for kind, name in ((SV.STATE_MW, "state"), (SV.QFIM, "qfim")):
+        lat = np.array([r.latency for r in res if r.kind == kind])
+        if lat.size:
+            st[name + "_p50_ms"] = float(np.percentile(lat, 50) * 1e3)
+            st[name + "_p99_ms"] = float(np.percentile(lat, 99) * 1e3)
     return st
 
 
