@@ -192,6 +192,11 @@ int pqs_upload(pqs_ctx *ctx, void *dst, const void *src, int64_t bytes, void *st
  * (reference circuit.py:118-125, gates.py:122-131). */
 int pqs_plan_create(int32_t n_qubits, const pqs_op *ops, int32_t n_ops,
                     const pqs_plan_opts *opts, pqs_plan **out);
+/* Frees the plan.  A plan that has run waits for the whole device to be idle
+ * first (work queued on any stream may still read its arrays), so a loop
+ * with other streams busy should keep its plans rather than drop them.
+ * A plan's arrays are copied to the device at its first launch, in that
+ * launch's stream order; launches on other streams wait for that copy. */
 int pqs_plan_destroy(pqs_plan *plan);
 /* Plan shape: number of sweeps, tile bits, ops after merging, tangents. */
 int pqs_plan_info(const pqs_plan *plan, int32_t *n_sweeps, int32_t *tile_bits,
