@@ -107,3 +107,16 @@ def test_clifford_circuit_layers_structure():
         assert len(singles) == (n // 2) * T.LEN_CLIFF_STRING
         assert {type(g).__name__ for g in singles} <= {"H", "S"}
         assert len({g.q_on for g in singles}) == n // 2
+
+
+def test_to_device_rejects_unsupported_dtypes_before_any_device_call():
+    """engine.to_device maps float32/64, complex64/128 and int32/64 arrays to
+    torch dtypes; any other dtype raises TypeError on the host, before the
+    engine asks for a device (so this runs without a GPU)."""
+    import pytest
+    from pyramaterised_amd import engine as E
+    for dt in (np.uint8, np.uint16, np.int16, np.float16, np.bool_):
+        with pytest.raises(TypeError):
+            E.to_device(np.zeros(3, dtype=dt))
+    assert set(E._NP_TO_TORCH) == {np.dtype(x) for x in (np.float64, np.float32, np.complex128, np.complex64,
+                                                          np.int64, np.int32)}
