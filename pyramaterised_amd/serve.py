@@ -273,7 +273,7 @@ class Server:
             # sweep of hundreds of rows (tens of milliseconds each)
             for s in self.slots[:1 if len(self.slots) == 2 else 2]:
                 s.ctx.set_max_blocks(qfim_launch_blocks)
-        self._next_slot = 0
+        self._next_slot = {}  # lane -> round-robin position among its streams
         self.circuits = {}
         self.queues = OrderedDict()  # class key -> deque of requests (FIFO)
         self.results = {}
@@ -458,9 +458,10 @@ class Server:
             s = cands[0]
             return s if len(s.inflight) < limit else None
         for k in range(len(cands)):
-            s = cands[(self._next_slot + k) % len(cands)]
+            start = self._next_slot.get(lane, 0)
+            s = cands[(start + k) % len(cands)]
             if len(s.inflight) < limit:
-                self._next_slot = (self._next_slot + k + 1) % len(cands)
+                self._next_slot[lane] = (start + k + 1) % len(cands)
                 return s
         return None
 

@@ -62,7 +62,7 @@ def test_stream_lanes_keep_kinds_apart():
 
     def server(n_streams, max_inflight=2):
         return types.SimpleNamespace(slots=[types.SimpleNamespace(inflight=deque(), i=i) for i in range(n_streams)],
-                                     max_inflight=max_inflight, _next_slot=0)
+                                     max_inflight=max_inflight, _next_slot={})
 
     pick = SV.Server._free_slot
     s1 = server(1)
@@ -77,8 +77,11 @@ def test_stream_lanes_keep_kinds_apart():
     assert pick(s2, "state") is None  # full: a state request waits rather than join the QFIM stream
     s4 = server(4)
     assert pick(s4, "large").i == 0 and pick(s4, "qfim").i == 1
-    got = [pick(s4, "state").i for _ in range(4)]
-    assert sorted(set(got)) == [2, 3]
+    got = []
+    for _ in range(4):  # QFIM admissions in between do not reset the state lane's rotation
+        got.append(pick(s4, "state").i)
+        pick(s4, "qfim")
+    assert got == [2, 3, 2, 3]
     s4.slots[1].inflight.extend([0, 0])
     assert pick(s4, "qfim") is None
     s4.slots[2].inflight.extend([0, 0])
